@@ -1,0 +1,1296 @@
+/*
+ * pp_oracle.c -- TEST INFRASTRUCTURE ONLY (see pp_oracle.h).
+ *
+ * FP64 CPU restatement of pplacer's placement path.  Every function cites the
+ * reference file:line it follows.  Built with -ffp-contract=off so that every
+ * `a += b * c` rounds twice, as the reference's C stubs do on x86-64 (SSE2, no
+ * FMA contraction) and as OCaml's float arithmetic does.
+ *
+ * Third-party arithmetic that pplacer takes from GSL (not vendored in the
+ * reference) is restated from the published methods:
+ *   - Brent minimisation: GSL 2.x `gsl_min_fminimizer_brent` (Brent 1973,
+ *     "Algorithms for Minimization without Derivatives", ch. 5), with GSL's
+ *     relative step tolerance sqrt(DBL_EPSILON)*|x| and golden 0.3819660;
+ *   - symmetric eigensolver (GSL `symmv`): cyclic Jacobi rotations;
+ *   - regularised incomplete gamma P(a,x) (GSL `sf_gamma_inc_P`): series /
+ *     Lentz continued fraction (Press et al., Numerical Recipes 6.2), and its
+ *     inverse (GSL `cdf_gamma_Pinv`) by bracketed bisection + Newton.
+ */
+#define _GNU_SOURCE
+#include "pp_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#define LOG_OF_2 0.69314718055994530942 /* OCaml `log 2.` (gstar_support.ml:7) */
+
+/* ======================================================================== */
+/* linear_c.c                                                                */
+/* ======================================================================== */
+
+/* linear_c.c:29-90 -- dst[site][i] = sum_j a[i][j] * b[site][j] */
+void or_gemmish(double *dst, const double *a, const double *b, int n_sites, int n_states)
+{
+  for (int site = 0; site < n_sites; site++) {
+    const double *arow = a;
+    for (int i = 0; i < n_states; i++) {
+      double acc = 0;
+      for (int j = 0; j < n_states; j++) acc += arow[j] * b[j];
+      *dst++ = acc;
+      arow += n_states;
+    }
+    b += n_states;
+  }
+}
+
+/* linear_c.c:92-146 -- dst[i][j] = sum_k lambda[k] * u[i][k] * uit[j][k] */
+void or_dediagonalize(double *dst, const double *u, const double *lambda,
+                      const double *uit, int n)
+{
+  for (int i = 0; i < n; i++) {
+    const double *uit_p = uit;
+    for (int j = 0; j < n; j++) {
+      double acc = 0;
+      for (int k = 0; k < n; k++) acc += lambda[k] * u[k] * uit_p[k];
+      *dst++ = acc;
+      uit_p += n;
+    }
+    u += n;
+  }
+}
+
+/* linear_c.c:367-424 */
+double or_ten_log_like3(const double *statd, const double *x, const double *y,
+                        const double *z, double *util, int n_rates, int n_sites,
+                        int n_states)
+{
+  for (int site = 0; site < n_sites; site++) util[site] = 0.0;
+  for (int rate = 0; rate < n_rates; rate++) {
+    double *util_v = util;
+    for (int site = 0; site < n_sites; site++) {
+      for (int s = 0; s < n_states; s++) *util_v += statd[s] * x[s] * y[s] * z[s];
+      x += n_states; y += n_states; z += n_states;
+      util_v++;
+    }
+  }
+  double ll_tot = 0;
+  for (int site = 0; site < n_sites; site++) ll_tot += log(util[site]);
+  ll_tot -= ((float)n_sites) * log((float)n_rates);
+  return ll_tot;
+}
+
+/* linear_c.c:426-441 */
+void or_ten_pairwise_prod(double *dst, const double *x, const double *y, long size)
+{
+  for (long i = 0; i < size; i++) dst[i] = x[i] * y[i];
+}
+
+/* linear_c.c:443-463 */
+void or_ten_statd_pairwise_prod(const double *statd, double *dst, const double *a,
+                                const double *b, int n_rates, int n_sites, int n_states)
+{
+  for (int rate = 0; rate < n_rates; rate++)
+    for (int site = 0; site < n_sites; site++)
+      for (int s = 0; s < n_states; s++) *dst++ = statd[s] * (*a++) * (*b++);
+}
+
+/* linear_c.c:465-539 */
+double or_ten_masked_logdot(const double *x, const double *y, const uint16_t *mask,
+                            double *util, int n_rates, int n_sites, int n_states)
+{
+  for (int site = 0; site < n_sites; site++) util[site] = 0.0;
+  for (int rate = 0; rate < n_rates; rate++) {
+    double *util_v = util;
+    const double *x_p = x + (long)rate * n_sites * n_states;
+    const double *y_p = y + (long)rate * n_sites * n_states;
+    for (int site = 0; site < n_sites; site++) {
+      if (mask[site])
+        for (int s = 0; s < n_states; s++) *util_v += x_p[s] * y_p[s];
+      x_p += n_states; y_p += n_states;
+      util_v++;
+    }
+  }
+  double ll_tot = 0;
+  int n_used = 0;
+  for (int site = 0; site < n_sites; site++)
+    if (mask[site]) { ll_tot += log(util[site]); n_used++; }
+  ll_tot -= ((float)n_used) * log((float)n_rates);
+  return ll_tot;
+}
+
+/* linear_c.c:541-605 */
+double or_ten_bounded_logdot(const double *x, const double *y, int first, int last,
+                             double *util, int n_rates, int n_sites, int n_states)
+{
+  int n_used = 1 + last - first;
+  for (int site = 0; site < n_used; site++) util[site] = 0.0;
+  for (int rate = 0; rate < n_rates; rate++) {
+    double *util_v = util;
+    long boost = (long)rate * n_sites * n_states + (long)first * n_states;
+    const double *x_p = x + boost, *y_p = y + boost;
+    for (int site = 0; site < n_used; site++) {
+      for (int s = 0; s < n_states; s++) *util_v += x_p[s] * y_p[s];
+      x_p += n_states; y_p += n_states;
+      util_v++;
+    }
+  }
+  double ll_tot = 0;
+  for (int site = 0; site < n_used; site++) ll_tot += log(util[site]);
+  ll_tot -= ((float)n_used) * log((float)n_rates);
+  return ll_tot;
+}
+
+/* linear_c.c:251-296 */
+double or_mat_masked_logdot(const double *x, const double *y, const uint16_t *mask,
+                            int n_sites, int n_states)
+{
+  double ll_tot = 0;
+  for (int site = 0; site < n_sites; site++) {
+    if (mask[site]) {
+      double util = 0;
+      for (int s = 0; s < n_states; s++) util += x[s] * y[s];
+      ll_tot += log(util);
+    }
+    x += n_states; y += n_states;
+  }
+  return ll_tot;
+}
+
+/* linear_c.c:170-216 */
+double or_mat_log_like3(const double *statd, const double *x, const double *y,
+                        const double *z, int n_sites, int n_states)
+{
+  double ll_tot = 0;
+  for (int site = 0; site < n_sites; site++) {
+    double util = 0;
+    for (int s = 0; s < n_states; s++) util += statd[s] * x[s] * y[s] * z[s];
+    ll_tot += log(util);
+    x += n_states; y += n_states; z += n_states;
+  }
+  return ll_tot;
+}
+
+/* ======================================================================== */
+/* gstar_support.ml                                                          */
+/* ======================================================================== */
+
+/* gstar_support.ml:33 (frexp exponent) */
+int or_get_twoexp(double x) { int e; (void)frexp(x, &e); return e; }
+/* gstar_support.ml:40 -- ldexp 0.5 (i+1) */
+double or_of_twoexp(int i) { return ldexp(0.5, i + 1); }
+
+/* gstar_support.ml:43-49 */
+double or_total_twoexp(const int64_t *e, int n)
+{
+  double tot = 0.;
+  for (int i = 0; i < n; i++) tot += (double)e[i];
+  return tot;
+}
+/* gstar_support.ml:52-59 */
+double or_masked_total_twoexp(const int64_t *e, const uint16_t *mask, int n)
+{
+  double tot = 0.;
+  for (int i = 0; i < n; i++) if (mask[i] != 0) tot += (double)e[i];
+  return tot;
+}
+/* gstar_support.ml:62-67 */
+double or_bounded_total_twoexp(const int64_t *e, int start, int last)
+{
+  double tot = 0.;
+  for (int i = start; i <= last; i++) tot += (double)e[i];
+  return tot;
+}
+
+/* gmix_model.ml:163-187 -- per site, max over rates of frexp(max state);
+ * if below threshold scale every rate by 2^-max and add max to e[site]. */
+void or_perhaps_pull_exponent(int min_allowed_twoexp, double *a, int64_t *e,
+                              int n_rates, int n_sites, int n_states)
+{
+  for (int site = 0; site < n_sites; site++) {
+    long max_twoexp = -(long)4611686018427387903L; /* -max_int (63-bit) */
+    for (int rate = 0; rate < n_rates; rate++) {
+      const double *s = a + ((long)rate * n_sites + site) * n_states;
+      /* Gsl.Vector.max: first maximum under `>` */
+      double mx = s[0];
+      for (int k = 1; k < n_states; k++) if (s[k] > mx) mx = s[k];
+      int te; (void)frexp(mx, &te);
+      if (te > max_twoexp) max_twoexp = te;
+    }
+    if (max_twoexp < min_allowed_twoexp) {
+      double f = or_of_twoexp(-(int)max_twoexp);
+      for (int rate = 0; rate < n_rates; rate++) {
+        double *s = a + ((long)rate * n_sites + site) * n_states;
+        for (int k = 0; k < n_states; k++) s[k] *= f;
+      }
+      e[site] += max_twoexp;
+    }
+  }
+}
+
+/* ======================================================================== */
+/* gamma.ml                                                                  */
+/* ======================================================================== */
+
+/* regularised lower incomplete gamma P(a, x) (GSL sf_gamma_inc_P) */
+double or_gamma_inc_P(double a, double x)
+{
+  if (x <= 0.0) return 0.0;
+  double lpre = -x + a * log(x) - lgamma(a);
+  if (x < a + 1.0) {
+    double ap = a, sum = 1.0 / a, del = sum;
+    for (int n = 0; n < 100000; n++) {
+      ap += 1.0;
+      del *= x / ap;
+      sum += del;
+      if (fabs(del) < fabs(sum) * 1e-17) break;
+    }
+    return sum * exp(lpre);
+  } else {
+    const double tiny = 1e-300;
+    double b = x + 1.0 - a, c = 1.0 / tiny, d = 1.0 / b, h = d;
+    for (int i = 1; i < 100000; i++) {
+      double an = -i * (i - a);
+      b += 2.0;
+      d = an * d + b; if (fabs(d) < tiny) d = tiny;
+      c = b + an / c; if (fabs(c) < tiny) c = tiny;
+      d = 1.0 / d;
+      double del = d * c;
+      h *= del;
+      if (fabs(del - 1.0) < 1e-17) break;
+    }
+    return 1.0 - exp(lpre) * h;
+  }
+}
+
+/* inverse of P(a, .) (GSL cdf_gamma_Pinv with scale 1) */
+double or_gamma_Pinv(double a, double p)
+{
+  if (p <= 0.0) return 0.0;
+  if (p >= 1.0) return INFINITY;
+  double lo = 0.0, hi = a > 1.0 ? a : 1.0;
+  while (or_gamma_inc_P(a, hi) < p) { lo = hi; hi *= 2.0; }
+  for (int it = 0; it < 400; it++) {
+    double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (or_gamma_inc_P(a, mid) < p) lo = mid; else hi = mid;
+  }
+  double x = 0.5 * (lo + hi);
+  /* polish with Newton steps kept inside the bracket */
+  for (int it = 0; it < 8; it++) {
+    double fx = or_gamma_inc_P(a, x) - p;
+    double dens = exp(-x + (a - 1.0) * log(x) - lgamma(a));
+    if (!(dens > 0)) break;
+    double nx = x - fx / dens;
+    if (!(nx > 0) || nx == x) break;
+    x = nx;
+  }
+  return x;
+}
+
+/* gamma.ml:14-26 -- inverse_gamma_cdf ~alpha ~beta:1 with epsilon 1e-7 */
+static double inverse_gamma_cdf(double alpha, double y)
+{
+  const double epsilon = 1e-7;
+  if (y < or_gamma_inc_P(alpha, epsilon)) return 0.;
+  return or_gamma_Pinv(alpha, y);
+}
+
+/* gamma.ml:52-70 -- discrete_gamma n_cat alpha (Yang 1994, mean rates) */
+int or_discrete_gamma(int n_cat, double alpha, double *rates)
+{
+  if (n_cat <= 0) return OR_BAD_INPUT;
+  double *freq = malloc(sizeof(double) * (n_cat + 1));
+  for (int i = 0; i <= n_cat; i++) {
+    if (i == 0) freq[i] = 0.;
+    else if (i == n_cat) freq[i] = 1.;
+    else {
+      double cut = inverse_gamma_cdf(alpha, (double)i / (double)n_cat);
+      freq[i] = or_gamma_inc_P(alpha + 1., cut);
+    }
+  }
+  /* gamma.ml:27-30 make_mean_one */
+  double tot = 0.;
+  for (int i = 0; i < n_cat; i++) tot += freq[i + 1] - freq[i];
+  double fn = (double)n_cat;
+  for (int i = 0; i < n_cat; i++) rates[i] = fn * (freq[i + 1] - freq[i]) / tot;
+  free(freq);
+  return OR_OK;
+}
+
+/* ======================================================================== */
+/* diagd.ml                                                                  */
+/* ======================================================================== */
+
+/* Symmetric eigendecomposition by cyclic Jacobi rotations.  l[k] is an
+ * eigenvalue and column k of u (u[i*n+k]) its eigenvector, as GSL symmv
+ * returns them (common_src/linear_utils.ml:206-208). */
+int or_symm_eigs(const double *m, int n, double *l, double *u)
+{
+  double *a = malloc(sizeof(double) * n * n);
+  memcpy(a, m, sizeof(double) * n * n);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) u[i * n + j] = (i == j) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0, diag = 0.0;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        if (i != j) off += a[i * n + j] * a[i * n + j];
+        else diag += a[i * n + j] * a[i * n + j];
+      }
+    if (off <= 1e-300 || off <= diag * 1e-34) break;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        double apq = a[p * n + q];
+        if (apq == 0.0) continue;
+        double app = a[p * n + p], aqq = a[q * n + q];
+        double theta = (aqq - app) / (2.0 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; k++) {
+          double akp = a[k * n + p], akq = a[k * n + q];
+          a[k * n + p] = c * akp - s * akq;
+          a[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          double apk = a[p * n + k], aqk = a[q * n + k];
+          a[p * n + k] = c * apk - s * aqk;
+          a[q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          double ukp = u[k * n + p], ukq = u[k * n + q];
+          u[k * n + p] = c * ukp - s * ukq;
+          u[k * n + q] = s * ukp + c * ukq;
+        }
+      }
+  }
+  for (int i = 0; i < n; i++) l[i] = a[i * n + i];
+  free(a);
+  return OR_OK;
+}
+
+static double safe_invert(double x) { return x == 0. ? 1. : 1. / x; } /* diagd.ml:118 */
+
+/* diagd.ml:140-154 */
+void or_b_of_exchangeable_pair(const double *r, const double *pi, int n, double *b)
+{
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      if (i != j) b[i * n + j] = r[i * n + j];
+      else {
+        double total = 0.;
+        for (int k = 0; k < n; k++)
+          if (k != i) total += r[k * n + i] * pi[k];
+        b[i * n + j] = -(total * safe_invert(pi[i]));
+      }
+    }
+}
+
+/* diagd.ml:123-132 -- M = D^1/2 B D^1/2 = U L U^T ; x = D^-1/2 U ; xit = D^1/2 U
+ * (the dgemm products with diagonal factors reduce to single products). */
+int or_of_d_b(const double *d, const double *b, int n, double *x, double *l, double *xit)
+{
+  for (int i = 0; i < n; i++) if (!(d[i] >= 0.)) return OR_BAD_INPUT;
+  double *droot = malloc(sizeof(double) * n), *m = malloc(sizeof(double) * n * n);
+  double *u = malloc(sizeof(double) * n * n);
+  for (int i = 0; i < n; i++) droot[i] = sqrt(d[i]);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) m[i * n + j] = droot[i] * (b[i * n + j] * droot[j]);
+  or_symm_eigs(m, n, l, u);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      x[i * n + j] = safe_invert(droot[i]) * u[i * n + j];
+      xit[i * n + j] = droot[i] * u[i * n + j];
+    }
+  free(droot); free(m); free(u);
+  return OR_OK;
+}
+
+/* diagd.ml:158-172 -- rescale so that -sum_i pi_i Q_ii = 1 */
+int or_normed_of_exchangeable_pair(const double *r, const double *pi, int n,
+                                   double *x, double *l, double *xit)
+{
+  double *b = malloc(sizeof(double) * n * n), *q = malloc(sizeof(double) * n * n);
+  or_b_of_exchangeable_pair(r, pi, n, b);
+  int st = or_of_d_b(pi, b, n, x, l, xit);
+  if (st == OR_OK) {
+    or_dediagonalize(q, x, l, xit, n);
+    double rate = 0.;
+    for (int i = 0; i < n; i++) rate = rate - q[i * n + i] * pi[i];
+    double s = 1. / rate;
+    for (int i = 0; i < n; i++) l[i] *= s;
+  }
+  free(b); free(q);
+  return st;
+}
+
+/* diagd.ml:65-71 */
+void or_to_exp(double *dst, const double *x, const double *l, const double *xit, int n,
+               double bl)
+{
+  double *util = malloc(sizeof(double) * n);
+  for (int i = 0; i < n; i++) util[i] = exp(bl * l[i]);
+  or_dediagonalize(dst, x, util, xit, n);
+  free(util);
+}
+
+/* diagd.ml:79-100 -- P_r(t) = X diag(exp(rate_r t l)) X^-1 for every rate */
+void or_multi_exp(double *dst, const double *x, const double *l, const double *xit,
+                  const double *rates, int n_rates, int n, double bl)
+{
+  double util[64];
+  memset(dst, 0, sizeof(double) * n_rates * n * n);
+  for (int r = 0; r < n_rates; r++) {
+    for (int i = 0; i < n; i++) util[i] = exp(rates[r] * bl * l[i]);
+    or_dediagonalize(dst + (long)r * n * n, x, util, xit, n);
+  }
+}
+
+/* ======================================================================== */
+/* tip encoding                                                              */
+/* ======================================================================== */
+
+/* nuc_models.ml:24-45, columns A C G T */
+static int nuc_bits(char c)
+{
+  switch (c) {
+  case 'A': return 0x1; case 'C': return 0x2; case 'G': return 0x4; case 'T': return 0x8;
+  case 'R': return 0x5; case 'Y': return 0xA; case 'M': return 0x3; case 'K': return 0xC;
+  case 'S': return 0x6; case 'W': return 0x9; case 'H': return 0xB; case 'B': return 0xE;
+  case 'V': return 0x7; case 'D': return 0xD;
+  case 'N': case '-': case '?': case 'X': return 0xF;
+  default: return -1;
+  }
+}
+/* prot_models.ml:5 (order) and :7-49 (ambiguity codes) */
+static const char PROT_CODE[20] = {'A','R','N','D','C','Q','E','G','H','I',
+                                   'L','K','M','F','P','S','T','W','Y','V'};
+static int prot_lv(char c, double *lv)
+{
+  for (int i = 0; i < 20; i++) lv[i] = 0.;
+  for (int i = 0; i < 20; i++) if (PROT_CODE[i] == c) { lv[i] = 1.; return 0; }
+  switch (c) {
+  case 'B': lv[3] = 1.; lv[2] = 1.; return 0;   /* D or N */
+  case 'J': lv[9] = 1.; lv[10] = 1.; return 0;  /* I or L */
+  case 'Z': lv[6] = 1.; lv[5] = 1.; return 0;   /* E or Q */
+  case '-': case '?': case 'X':
+    for (int i = 0; i < 20; i++) lv[i] = 1.;
+    return 0;
+  default: return -1;
+  }
+}
+
+int or_lv_of_char(int n_states, char c, double *lv)
+{
+  if (n_states == 4) {
+    int b = nuc_bits(c);
+    if (b < 0) return OR_BAD_INPUT;
+    for (int i = 0; i < 4; i++) lv[i] = (b >> i) & 1 ? 1. : 0.;
+    return OR_OK;
+  }
+  if (n_states == 20) return prot_lv(c, lv) ? OR_BAD_INPUT : OR_OK;
+  return OR_BAD_INPUT;
+}
+
+/* alignment.ml:235 */
+int or_informative(char c) { return !(c == '?' || c == '-'); }
+
+/* core.ml:25-28 */
+void or_mask_reindex(const uint8_t *mask, int n, int *out, int *n_out)
+{
+  int k = 0;
+  for (int i = 0; i < n; i++) if (mask[i]) out[k++] = i;
+  *n_out = k;
+}
+
+/* alignment.ml:186-217 -- empirical frequencies over non-gap characters */
+int or_emper_freq(int n_states, const char *seqs, int n_seq, int n_sites, double *freq)
+{
+  double lv[20];
+  for (int i = 0; i < n_states; i++) freq[i] = 0.;
+  for (int s = 0; s < n_seq; s++)
+    for (int k = 0; k < n_sites; k++) {
+      char c = seqs[(long)s * n_sites + k];
+      if (c == '-' || c == '?') continue;
+      if (or_lv_of_char(n_states, c, lv) != OR_OK) return OR_BAD_INPUT;
+      double tot = 0.;                          /* linear_utils.ml:84-105 */
+      for (int i = 0; i < n_states; i++) tot += lv[i];
+      double sc = 1. / tot;
+      for (int i = 0; i < n_states; i++) freq[i] += lv[i] * sc;
+    }
+  double tot = 0.;
+  for (int i = 0; i < n_states; i++) tot += freq[i];
+  double sc = 1. / tot;
+  for (int i = 0; i < n_states; i++) freq[i] *= sc;
+  return OR_OK;
+}
+
+/* pplacer_run.ml:59-144 */
+int or_premask(int n_states, const char *ref_seqs, int n_ref, const char *query_seqs,
+               int n_query, int n_sites, int discard_nonoverlapped, uint8_t *mask,
+               uint8_t *query_keep, int *masklen)
+{
+  double lv[20];
+  uint8_t *ref_mask = calloc(n_sites, 1), *q_mask = calloc(n_sites, 1);
+  int st = OR_OK;
+  for (int s = 0; s < n_ref && st == OR_OK; s++)
+    for (int k = 0; k < n_sites; k++) {
+      char c = ref_seqs[(long)s * n_sites + k];
+      if (or_lv_of_char(n_states, c, lv) != OR_OK) { st = OR_BAD_INPUT; break; }
+      if (or_informative(c)) ref_mask[k] = 1;
+    }
+  for (int q = 0; q < n_query && st == OR_OK; q++) {
+    const char *seq = query_seqs + (long)q * n_sites;
+    int overlaps = 0;
+    for (int k = 0; k < n_sites; k++)
+      if (or_informative(seq[k]) && ref_mask[k]) { overlaps = 1; break; }
+    if (!overlaps && !discard_nonoverlapped) { st = OR_BAD_INPUT; break; }
+    query_keep[q] = (uint8_t)overlaps;
+    if (!overlaps) continue;
+    for (int k = 0; k < n_sites; k++) {
+      char c = seq[k];
+      if (or_lv_of_char(n_states, c, lv) != OR_OK) { st = OR_BAD_INPUT; break; }
+      if (or_informative(c)) q_mask[k] = 1;
+    }
+  }
+  int n = 0;
+  for (int k = 0; k < n_sites; k++) { mask[k] = ref_mask[k] && q_mask[k]; n += mask[k]; }
+  *masklen = n;
+  if (st == OR_OK && n == 0) st = OR_BAD_INPUT;
+  free(ref_mask); free(q_mask);
+  return st;
+}
+
+/* ======================================================================== */
+/* gmix_model.ml GLV operations                                              */
+/* ======================================================================== */
+
+/* gmix_model.ml:308-315 -> gstar_support.ml:70-79 -> linear_c.c:29-90 */
+static void evolve_with(const or_model *m, double *tensor, double *dst_a, int64_t *dst_e,
+                        const double *src_a, const int64_t *src_e, int n_sites, double bl)
+{
+  int R = m->n_rates, S = m->n_states;
+  memcpy(dst_e, src_e, sizeof(int64_t) * n_sites);
+  or_multi_exp(tensor, m->x, m->l, m->xit, m->rates, R, S, bl);
+  for (int r = 0; r < R; r++)
+    or_gemmish(dst_a + (long)r * n_sites * S, tensor + (long)r * S * S,
+               src_a + (long)r * n_sites * S, n_sites, S);
+}
+
+void or_evolve_into(const or_model *m, double *dst_a, int64_t *dst_e,
+                    const double *src_a, const int64_t *src_e, int n_sites, double bl)
+{
+  double *tensor = malloc(sizeof(double) * m->n_rates * m->n_states * m->n_states);
+  evolve_with(m, tensor, dst_a, dst_e, src_a, src_e, n_sites, bl);
+  free(tensor);
+}
+
+/* gmix_model.ml:298-304 */
+double or_log_like3(const or_model *m, double *util, int n_sites,
+                    const double *xa, const int64_t *xe, const double *ya,
+                    const int64_t *ye, const double *za, const int64_t *ze)
+{
+  return or_ten_log_like3(m->statd, xa, ya, za, util, m->n_rates, n_sites, m->n_states)
+         + (LOG_OF_2 * ((or_total_twoexp(xe, n_sites) + or_total_twoexp(ye, n_sites))
+                        + or_total_twoexp(ze, n_sites)));
+}
+
+/* gmix_model.ml:194-198 */
+double or_masked_logdot(double *util, int n_rates, int n_sites, int n_states,
+                        const double *xa, const int64_t *xe, const double *ya,
+                        const int64_t *ye, const uint16_t *mask)
+{
+  return or_ten_masked_logdot(xa, ya, mask, util, n_rates, n_sites, n_states)
+         + (LOG_OF_2 * (or_masked_total_twoexp(xe, mask, n_sites)
+                        + or_masked_total_twoexp(ye, mask, n_sites)));
+}
+
+/* gmix_model.ml:202-212 */
+double or_logdot(double *util, int n_rates, int n_sites, int n_states,
+                 const double *xa, const int64_t *xe, const double *ya,
+                 const int64_t *ye)
+{
+  return or_ten_bounded_logdot(xa, ya, 0, n_sites - 1, util, n_rates, n_sites, n_states)
+         + (LOG_OF_2 * (or_bounded_total_twoexp(xe, 0, n_sites - 1)
+                        + or_bounded_total_twoexp(ye, 0, n_sites - 1)));
+}
+
+/* gmix_model.ml:325-351 */
+double or_slow_log_like3(const or_model *m, int n_sites,
+                         const double *xa, const int64_t *xe, const double *ya,
+                         const int64_t *ye, const double *za, const int64_t *ze)
+{
+  int R = m->n_rates, S = m->n_states;
+  double f_n_rates = (double)R, tot = 0.;
+  for (int site = 0; site < n_sites; site++) {
+    double site_like = 0.;
+    for (int r = 0; r < R; r++)
+      for (int s = 0; s < S; s++) {
+        long o = ((long)r * n_sites + site) * S + s;
+        site_like = site_like + m->statd[s] * xa[o] * ya[o] * za[o];
+      }
+    double v = log(site_like / f_n_rates)
+               + LOG_OF_2 * (double)(xe[site] + ye[site] + ze[site]);
+    tot = tot + v;
+  }
+  return tot;
+}
+
+/* ======================================================================== */
+/* like_stree.ml / glv_arr.ml                                                */
+/* ======================================================================== */
+
+typedef struct { double *a; int64_t *e; } glv;
+
+static glv glv_at(double *arr, int64_t *arr_e, int i, long glv_len, int n_sites)
+{
+  glv g = { arr + (long)i * glv_len, arr_e + (long)i * n_sites };
+  return g;
+}
+
+/* gmix_model.ml:215-218 */
+static void pairwise_prod(glv dst, glv x, glv y, long glv_len, int n_sites)
+{
+  for (int i = 0; i < n_sites; i++) dst.e[i] = x.e[i] + y.e[i];
+  or_ten_pairwise_prod(dst.a, x.a, y.a, glv_len);
+}
+
+/* gmix_model.ml:224-233 */
+static void listwise_prod(glv dst, glv *xs, int n, long glv_len, int n_sites)
+{
+  if (n >= 2) {
+    pairwise_prod(dst, xs[0], xs[1], glv_len, n_sites);
+    for (int k = 2; k < n; k++) pairwise_prod(dst, dst, xs[k], glv_len, n_sites);
+  } else if (n == 1) {
+    memcpy(dst.e, xs[0].e, sizeof(int64_t) * n_sites);
+    memcpy(dst.a, xs[0].a, sizeof(double) * glv_len);
+  }
+}
+
+/* gmix_model.ml:149-156 */
+static int prep_constant_rate(glv g, const char *seq, int n_rates, int n_sites, int S)
+{
+  double lv[20];
+  for (int site = 0; site < n_sites; site++) {
+    if (or_lv_of_char(S, seq[site], lv) != OR_OK) return OR_BAD_INPUT;
+    for (int r = 0; r < n_rates; r++)
+      memcpy(g.a + ((long)r * n_sites + site) * S, lv, sizeof(double) * S);
+    g.e[site] = 0;
+  }
+  return OR_OK;
+}
+
+#define MIN_ALLOWED_TWOEXP (-50) /* like_stree.ml:8 */
+
+/* like_stree.ml:43-69 (post-order) and :74-100 (pre-order) */
+int or_calc_distal_and_proximal(const or_model *m, const or_tree *t,
+                                const char *leaf_seqs, int n_sites,
+                                double *darr, int64_t *darr_e,
+                                double *parr, int64_t *parr_e,
+                                double *util_arr, int64_t *util_arr_e)
+{
+  int R = m->n_rates, S = m->n_states, G = t->n_nodes;
+  long L = (long)R * n_sites * S;
+  double *tensor = malloc(sizeof(double) * R * S * S);
+  glv *xs = malloc(sizeof(glv) * (G + 1));
+  int st = OR_OK;
+  /* post-order: ids are post-order, so increasing id visits children first */
+  for (int id = 0; id < G && st == OR_OK; id++) {
+    glv distal = glv_at(darr, darr_e, id, L, n_sites);
+    int c0 = t->child_start[id], c1 = t->child_start[id + 1];
+    if (c1 > c0) {
+      for (int k = c0; k < c1; k++) xs[k - c0] = glv_at(util_arr, util_arr_e, t->children[k], L, n_sites);
+      listwise_prod(distal, xs, c1 - c0, L, n_sites);
+      or_perhaps_pull_exponent(MIN_ALLOWED_TWOEXP, distal.a, distal.e, R, n_sites, S);
+    } else {
+      st = prep_constant_rate(distal, leaf_seqs + (long)id * n_sites, R, n_sites, S);
+    }
+    glv ev = glv_at(util_arr, util_arr_e, id, L, n_sites);
+    evolve_with(m, tensor, ev.a, ev.e, distal.a, distal.e, n_sites, t->bl[id]);
+  }
+  if (st == OR_OK) {
+    /* pre-order: root proximal is the unit GLV */
+    glv top = glv_at(parr, parr_e, G - 1, L, n_sites);
+    for (long i = 0; i < L; i++) top.a[i] = 1.;
+    for (int i = 0; i < n_sites; i++) top.e[i] = 0;
+    glv evolved_prox = { malloc(sizeof(double) * L), malloc(sizeof(int64_t) * n_sites) };
+    for (int id = G - 1; id >= 0; id--) {
+      int c0 = t->child_start[id], c1 = t->child_start[id + 1];
+      if (c1 == c0) continue;
+      glv prox = glv_at(parr, parr_e, id, L, n_sites);
+      evolve_with(m, tensor, evolved_prox.a, evolved_prox.e, prox.a, prox.e, n_sites, t->bl[id]);
+      for (int k = c0; k < c1; k++) {  /* ListFuns.pull_each_out: rest keeps order */
+        int n = 0;
+        xs[n++] = evolved_prox;
+        for (int j = c0; j < c1; j++)
+          if (j != k) xs[n++] = glv_at(util_arr, util_arr_e, t->children[j], L, n_sites);
+        glv below = glv_at(parr, parr_e, t->children[k], L, n_sites);
+        listwise_prod(below, xs, n, L, n_sites);
+        or_perhaps_pull_exponent(MIN_ALLOWED_TWOEXP, below.a, below.e, R, n_sites, S);
+      }
+    }
+    free(evolved_prox.a); free(evolved_prox.e);
+  }
+  free(tensor); free(xs);
+  return st;
+}
+
+/* pplacer_run.ml:331-336 */
+void or_pull_exponents_arr(int min_twoexp, double *arr, int64_t *arr_e, int n_glvs,
+                           int n_rates, int n_sites, int n_states)
+{
+  long L = (long)n_rates * n_sites * n_states;
+  for (int i = 0; i < n_glvs; i++)
+    or_perhaps_pull_exponent(min_twoexp, arr + i * L, arr_e + (long)i * n_sites,
+                             n_rates, n_sites, n_states);
+}
+
+/* glv_arr.ml:46-56 with half_bl_fun (pplacer_run.ml:337) */
+void or_prep_supernodes(const or_model *m, const or_tree *t, int n_sites,
+                        const double *darr, const int64_t *darr_e,
+                        const double *parr, const int64_t *parr_e,
+                        double *snodes, int64_t *snodes_e)
+{
+  int R = m->n_rates, S = m->n_states, G = t->n_nodes;
+  long L = (long)R * n_sites * S;
+  double *tensor = malloc(sizeof(double) * R * S * S);
+  double *ud = malloc(sizeof(double) * L), *up = malloc(sizeof(double) * L);
+  int64_t *ude = malloc(sizeof(int64_t) * n_sites), *upe = malloc(sizeof(int64_t) * n_sites);
+  for (int i = 0; i < G; i++) {
+    double half = t->bl[i] / 2.;
+    evolve_with(m, tensor, ud, ude, darr + i * L, darr_e + (long)i * n_sites, n_sites, half);
+    evolve_with(m, tensor, up, upe, parr + i * L, parr_e + (long)i * n_sites, n_sites, half);
+    for (int k = 0; k < n_sites; k++) snodes_e[(long)i * n_sites + k] = ude[k] + upe[k];
+    or_ten_statd_pairwise_prod(m->statd, snodes + i * L, ud, up, R, n_sites, S);
+  }
+  free(tensor); free(ud); free(up); free(ude); free(upe);
+}
+
+/* ======================================================================== */
+/* minimization.ml + GSL Brent                                               */
+/* ======================================================================== */
+
+#define SQRT_DBL_EPSILON 1.4901161193847656e-08
+
+/* minimization.ml:83-100 with bisection_start_finder (:11-41) and the GSL
+ * fminimizer (set: f(lower), f(upper), f(minimum); init: f(v); iterate). */
+int or_brent(or_fun f, void *ctx, double raw_start, double left, double right,
+             double tolerance, int max_iters, double *result)
+{
+  /* start_finder_prep, minimization.ml:11-17 */
+  if (left >= raw_start || raw_start >= right) return OR_INVALID_START;
+  double righty = f(right, ctx);   /* tuple (f left, f right): right-to-left */
+  double lefty = f(left, ctx);
+  double smaller = lefty < righty ? left : right;
+  double miny = lefty <= righty ? lefty : righty;   /* Stdlib.min */
+  double prev = raw_start, start;
+  int iter = 1;
+  for (;;) {   /* bisection_start_finder, minimization.ml:24-41 */
+    double pv = f(prev, ctx);
+    if (pv < miny) { start = prev; break; }
+    else if (fabs(prev - smaller) < tolerance) {
+      *result = (pv < miny) ? prev : smaller;   /* FoundMin */
+      return OR_OK;
+    } else if (iter > max_iters) return OR_FIND_START_FAILURE;
+    prev = (prev + smaller) / 2.;
+    iter++;
+  }
+  /* gsl_min_fminimizer_set */
+  double x_lower = left, x_upper = right, x_min = start;
+  double f_lower = f(x_lower, ctx); if (!isfinite(f_lower)) return OR_GSL_EBADFUNC;
+  double f_upper = f(x_upper, ctx); if (!isfinite(f_upper)) return OR_GSL_EBADFUNC;
+  double f_min = f(x_min, ctx); if (!isfinite(f_min)) return OR_GSL_EBADFUNC;
+  if (x_lower > x_upper) return OR_GSL_EINVAL;
+  if (x_min >= x_upper || x_min <= x_lower) return OR_GSL_EINVAL;
+  if (f_min >= f_lower || f_min >= f_upper) return OR_GSL_EINVAL;
+  /* Brent init */
+  const double golden = 0.3819660;
+  double v = x_lower + golden * (x_upper - x_lower), w = v;
+  double sd = 0., se = 0.;   /* state d, e */
+  double f_v = f(v, ctx); if (!isfinite(f_v)) return OR_GSL_EBADFUNC;
+  double f_w = f_v;
+  for (int step = 1;; step++) {
+    if (step > max_iters) return OR_EXCEEDED_MAX_ITER;
+    /* Brent iterate */
+    const double x_left = x_lower, x_right = x_upper, z = x_min, f_z = f_min;
+    double d = se, e = sd, u, f_u;
+    const double w_lower = z - x_left, w_upper = x_right - z;
+    const double tol = SQRT_DBL_EPSILON * fabs(z);
+    double p = 0, q = 0, r = 0;
+    const double midpoint = 0.5 * (x_left + x_right);
+    if (fabs(e) > tol) {
+      r = (z - w) * (f_z - f_v);
+      q = (z - v) * (f_z - f_w);
+      p = (z - v) * q - (z - w) * r;
+      q = 2 * (q - r);
+      if (q > 0) p = -p; else q = -q;
+      r = e;
+      e = d;
+    }
+    if (fabs(p) < fabs(0.5 * q * r) && p < q * w_lower && p < q * w_upper) {
+      double t2 = 2 * tol;
+      d = p / q;
+      u = z + d;
+      if ((u - x_left) < t2 || (x_right - u) < t2) d = (z < midpoint) ? tol : -tol;
+    } else {
+      e = (z < midpoint) ? x_right - z : -(z - x_left);
+      d = golden * e;
+    }
+    if (fabs(d) >= tol) u = z + d;
+    else u = z + ((d > 0) ? tol : -tol);
+    se = e; sd = d;
+    f_u = f(u, ctx);
+    if (!isfinite(f_u)) return OR_GSL_EBADFUNC;
+    if (f_u <= f_z) {
+      if (u < z) { x_upper = z; f_upper = f_z; }
+      else { x_lower = z; f_lower = f_z; }
+      v = w; f_v = f_w; w = z; f_w = f_z;
+      x_min = u; f_min = f_u;
+    } else {
+      if (u < z) { x_lower = u; f_lower = f_u; }
+      else { x_upper = u; f_upper = f_u; }
+      if (f_u <= f_w || w == z) { v = w; f_v = f_w; w = u; f_w = f_u; }
+      else if (f_u <= f_v || v == z || v == w) { v = u; f_v = f_u; }
+    }
+    (void)f_lower; (void)f_upper;
+    if (x_upper - x_lower < tolerance) { *result = x_min; return OR_OK; }
+  }
+}
+
+/* ======================================================================== */
+/* three_tax.ml + core.ml                                                    */
+/* ======================================================================== */
+
+typedef struct {
+  const or_model *m;
+  int M;              /* masked sites */
+  long L;             /* R*M*S */
+  double *tensor;
+  double *util;
+  glv prox_o, dist_o, pend_o, prox_v, dist_v, pend_v;
+  double prox_bl, dist_bl, pend_bl;
+  long n_like_calls;
+} three_tax;
+
+static void tt_set_edge_bl(three_tax *tt, glv orig, glv evolv, double *blp, double bl)
+{
+  *blp = bl;   /* glv_edge.ml:29-44 set_bl -> recalculate */
+  evolve_with(tt->m, tt->tensor, evolv.a, evolv.e, orig.a, orig.e, tt->M, bl);
+}
+
+/* three_tax.ml:30-36 */
+static double tt_log_like(three_tax *tt)
+{
+  return or_log_like3(tt->m, tt->util, tt->M, tt->prox_v.a, tt->prox_v.e,
+                      tt->dist_v.a, tt->dist_v.e, tt->pend_v.a, tt->pend_v.e);
+}
+
+typedef struct { three_tax *tt; int bad; } opt_ctx;
+
+/* three_tax.ml:38-39 + :51-55 */
+static double opt_pend(double v, void *c)
+{
+  three_tax *tt = ((opt_ctx *)c)->tt;
+  tt->n_like_calls++;
+  tt_set_edge_bl(tt, tt->pend_o, tt->pend_v, &tt->pend_bl, v);
+  return -tt_log_like(tt);
+}
+/* three_tax.ml:41-45 + :51-55 */
+static double opt_dist(double v, void *c)
+{
+  opt_ctx *oc = (opt_ctx *)c;
+  three_tax *tt = oc->tt;
+  tt->n_like_calls++;
+  double prox_bl = (tt->dist_bl + tt->prox_bl) - v;
+  if (!(prox_bl >= 0.)) oc->bad = 1;   /* assert(prox_bl >= 0.) */
+  tt_set_edge_bl(tt, tt->dist_o, tt->dist_v, &tt->dist_bl, v);
+  tt_set_edge_bl(tt, tt->prox_o, tt->prox_v, &tt->prox_bl, prox_bl);
+  return -tt_log_like(tt);
+}
+
+/* three_tax.ml:48-58 */
+static int optimize_something(three_tax *tt, double tolerance, int dist, double start_v,
+                              double min_v, double max_v, double *res)
+{
+  if (min_v == max_v) { *res = start_v; return OR_OK; }
+  opt_ctx oc = { tt, 0 };
+  int st = or_brent(dist ? opt_dist : opt_pend, &oc, start_v, min_v, max_v, tolerance,
+                    100, res);
+  if (oc.bad) return OR_BAD_INPUT;
+  return st;
+}
+
+#define CORE_MAX_ITER 200 /* core.ml:10 */
+#define FINAL_TOLERANCE 1e-5 /* core.ml:14 */
+
+/* three_tax.ml:70-83 */
+static int tt_optimize(three_tax *tt, double tolerance, double max_pend_bl, int max_iter)
+{
+  tt->n_like_calls = 0;
+  double prev_pend = tt->pend_bl, prev_dist = tt->dist_bl;
+  for (int which_step = 1;; which_step++) {
+    if (which_step > max_iter) return OR_EXCEEDED_MAX_ITER;
+    double curr_pend, curr_dist;
+    int st = optimize_something(tt, tolerance, 0, tt->pend_bl, 1e-8, max_pend_bl, &curr_pend);
+    if (st != OR_OK) return st;
+    st = optimize_something(tt, tolerance, 1, tt->dist_bl, 0., tt->dist_bl + tt->prox_bl,
+                            &curr_dist);
+    if (st != OR_OK) return st;
+    if (fabs(prev_pend - curr_pend) > tolerance || fabs(prev_dist - curr_dist) > tolerance) {
+      prev_pend = curr_pend; prev_dist = curr_dist;
+    } else return OR_OK;
+  }
+}
+
+/* gmix_model.ml:121-141 */
+static void mask_into(const uint8_t *mask, int N, glv src, glv dst, int R, int M, int S)
+{
+  int di = 0;
+  for (int si = 0; si < N; si++) {
+    if (!mask[si]) continue;
+    dst.e[di] = src.e[si];
+    for (int r = 0; r < R; r++)
+      memcpy(dst.a + ((long)r * M + di) * S, src.a + ((long)r * N + si) * S, sizeof(double) * S);
+    di++;
+  }
+}
+
+typedef struct {
+  const or_core *c;
+  three_tax tt;
+  const uint8_t *mask;
+} query_state;
+
+/* core.ml:225-239 */
+static void set_tt_edges(query_state *qs, int loc, double pendant, double distal)
+{
+  const or_core *c = qs->c;
+  int R = c->model->n_rates, S = c->model->n_states, N = c->n_sites;
+  long LN = (long)R * N * S;
+  three_tax *tt = &qs->tt;
+  double cut_bl = c->tree->bl[loc];
+  tt_set_edge_bl(tt, tt->pend_o, tt->pend_v, &tt->pend_bl, pendant);
+  glv d = { (double *)c->darr + loc * LN, (int64_t *)c->darr_e + (long)loc * N };
+  glv p = { (double *)c->parr + loc * LN, (int64_t *)c->parr_e + (long)loc * N };
+  mask_into(qs->mask, N, d, tt->dist_o, R, tt->M, S);
+  tt_set_edge_bl(tt, tt->dist_o, tt->dist_v, &tt->dist_bl, distal);
+  mask_into(qs->mask, N, p, tt->prox_o, R, tt->M, S);
+  tt_set_edge_bl(tt, tt->prox_o, tt->prox_v, &tt->prox_bl, cut_bl - distal);
+}
+
+static int is_gsl(int st) { return st == OR_GSL_EBADFUNC || st == OR_GSL_EINVAL; }
+
+/* core.ml:255-281: ml_optimize_location + safe_ml_optimize_location */
+static int safe_ml_optimize(query_state *qs, double tol, int loc, double *ll, double *pend,
+                            double *dist, long *calls)
+{
+  const or_core *c = qs->c;
+  three_tax *tt = &qs->tt;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    int st = tt_optimize(tt, tol, c->prefs.max_pend, CORE_MAX_ITER);
+    *calls += tt->n_like_calls;
+    if (st == OR_OK || st == OR_EXCEEDED_MAX_ITER) {
+      *ll = tt_log_like(tt);
+      *pend = tt->pend_bl;
+      *dist = tt->dist_bl;
+      return st;
+    }
+    if (!is_gsl(st) || attempt == 1) return st;
+    set_tt_edges(qs, loc, c->prefs.start_pend, c->tree->bl[loc] / 2.);  /* tt_edges_default */
+  }
+  return OR_OK;
+}
+
+static int alloc_tt(three_tax *tt, const or_model *m, int M)
+{
+  int R = m->n_rates, S = m->n_states;
+  tt->m = m; tt->M = M; tt->L = (long)R * M * S;
+  tt->tensor = malloc(sizeof(double) * R * S * S);
+  tt->util = malloc(sizeof(double) * (M > 0 ? M : 1));
+  glv *gs[6] = { &tt->prox_o, &tt->dist_o, &tt->pend_o, &tt->prox_v, &tt->dist_v, &tt->pend_v };
+  for (int i = 0; i < 6; i++) {
+    gs[i]->a = calloc(tt->L > 0 ? tt->L : 1, sizeof(double));
+    gs[i]->e = calloc(M > 0 ? M : 1, sizeof(int64_t));
+  }
+  tt->n_like_calls = 0;
+  return OR_OK;
+}
+static void free_tt(three_tax *tt)
+{
+  free(tt->tensor); free(tt->util);
+  glv *gs[6] = { &tt->prox_o, &tt->dist_o, &tt->pend_o, &tt->prox_v, &tt->dist_v, &tt->pend_v };
+  for (int i = 0; i < 6; i++) { free(gs[i]->a); free(gs[i]->e); }
+}
+
+/* OCaml `compare` on floats for a descending stable sort: nan is smallest */
+static int cmp_desc(double a, double b)
+{
+  int an = isnan(a), bn = isnan(b);
+  if (an || bn) return an && bn ? 0 : (an ? 1 : -1);
+  return a > b ? -1 : (a < b ? 1 : 0);
+}
+
+typedef struct { double key; int idx; } keyed;
+
+static void merge_sort_desc(keyed *v, keyed *tmp, int n)
+{
+  if (n < 2) return;
+  int h = n / 2;
+  merge_sort_desc(v, tmp, h);
+  merge_sort_desc(v + h, tmp, n - h);
+  int i = 0, j = h, k = 0;
+  while (i < h && j < n) tmp[k++] = (cmp_desc(v[j].key, v[i].key) < 0) ? v[j++] : v[i++];
+  while (i < h) tmp[k++] = v[i++];
+  while (j < n) tmp[k++] = v[j++];
+  memcpy(v, tmp, sizeof(keyed) * n);
+}
+
+/* fig.ml:143-147 (Dummy): stable sort of all non-root ids by score, descending */
+int or_rank(const double *scores, int n_edges, int *order)
+{
+  keyed *v = malloc(sizeof(keyed) * n_edges), *tmp = malloc(sizeof(keyed) * n_edges);
+  for (int i = 0; i < n_edges; i++) { v[i].key = scores[i]; v[i].idx = i; }
+  merge_sort_desc(v, tmp, n_edges);
+  for (int i = 0; i < n_edges; i++) order[i] = v[i].idx;
+  free(v); free(tmp);
+  return OR_OK;
+}
+
+/* core.ml:131-141 + :187-194: full-length query evolved by start_pend and the
+ * masked logdot against every supernode. */
+int or_stage1_scores(const or_core *c, const char *query, double *scores)
+{
+  const or_model *m = c->model;
+  int R = m->n_rates, S = m->n_states, N = c->n_sites, G = c->tree->n_nodes;
+  long LN = (long)R * N * S;
+  glv qo = { malloc(sizeof(double) * LN), malloc(sizeof(int64_t) * N) };
+  glv qe = { malloc(sizeof(double) * LN), malloc(sizeof(int64_t) * N) };
+  uint16_t *mask = malloc(sizeof(uint16_t) * N);
+  double *util = malloc(sizeof(double) * N);
+  int st = prep_constant_rate(qo, query, R, N, S);
+  if (st == OR_OK) {
+    or_evolve_into(m, qe.a, qe.e, qo.a, qo.e, N, c->prefs.start_pend);
+    for (int i = 0; i < N; i++) mask[i] = (uint16_t)or_informative(query[i]);
+    for (int loc = 0; loc < G - 1; loc++)
+      scores[loc] = or_masked_logdot(util, R, N, S, qe.a, qe.e, c->snodes + loc * LN,
+                                     c->snodes_e + (long)loc * N, mask);
+  }
+  free(qo.a); free(qo.e); free(qe.a); free(qe.e); free(mask); free(util);
+  return st;
+}
+
+static int setup_query(query_state *qs, const or_core *c, const char *query, uint8_t *mask)
+{
+  const or_model *m = c->model;
+  int R = m->n_rates, S = m->n_states, N = c->n_sites;
+  int M = 0;
+  for (int i = 0; i < N; i++) { mask[i] = (uint8_t)or_informative(query[i]); M += mask[i]; }
+  if (M == 0) return OR_BAD_INPUT;   /* "has no informative sites" */
+  qs->c = c; qs->mask = mask;
+  alloc_tt(&qs->tt, m, M);
+  /* query_glv: constant-rate GLV of the masked characters (core.ml:166-170) */
+  char *mq = malloc(M);
+  for (int i = 0, k = 0; i < N; i++) if (mask[i]) mq[k++] = query[i];
+  int st = prep_constant_rate(qs->tt.pend_o, mq, R, M, S);
+  free(mq);
+  if (st != OR_OK) { free_tt(&qs->tt); return st; }
+  qs->tt.pend_bl = qs->tt.dist_bl = qs->tt.prox_bl = c->prefs.start_pend;
+  return OR_OK;
+}
+
+int or_three_tax_optimize(const or_core *c, const char *query, int loc, double pend,
+                          double distal, double tolerance, double *ll,
+                          double *pend_out, double *dist_out, long *n_calls)
+{
+  uint8_t *mask = malloc(c->n_sites);
+  query_state qs;
+  int st = setup_query(&qs, c, query, mask);
+  if (st == OR_OK) {
+    set_tt_edges(&qs, loc, pend, distal);
+    *n_calls = 0;
+    st = safe_ml_optimize(&qs, tolerance, loc, ll, pend_out, dist_out, n_calls);
+    free_tt(&qs.tt);
+  }
+  free(mask);
+  return st;
+}
+
+/* ppatteries.ml:175-184 */
+void or_ll_normalized_prob(const double *ll, int n, double *out)
+{
+  for (int i = 0; i < n; i++) {
+    double tot = 0.;
+    for (int j = 0; j < n; j++) tot = tot + exp(ll[j] - ll[i]);
+    out[i] = 1. / tot;
+  }
+}
+
+static double now_s(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+typedef struct { int loc; double ll, pend, dist; } ml_result;
+
+/* core.ml:113-408 (no fantasy, no posterior probabilities, Dummy figs) */
+int or_process_query(const or_core *c, const char *query, or_placement *out,
+                     int *n_out, or_stats *stats)
+{
+  const or_prefs *pr = &c->prefs;
+  int G = c->tree->n_nodes, E = G - 1, N = c->n_sites;
+  or_stats local = {0};
+  or_stats *st_ = stats ? stats : &local;
+  *n_out = 0;
+  double t0 = now_s();
+  double *scores = malloc(sizeof(double) * E);
+  int *order = malloc(sizeof(int) * E);
+  int st = or_stage1_scores(c, query, scores);
+  if (st != OR_OK) { free(scores); free(order); return st; }
+  st_->n_logdots += E;
+  or_rank(scores, E, order);
+  double t1 = now_s();
+  st_->t_ranking += t1 - t0;
+
+  uint8_t *mask = malloc(N);
+  query_state qs;
+  st = setup_query(&qs, c, query, mask);
+  if (st != OR_OK) { free(scores); free(order); free(mask); return st; }
+
+  /* core.ml:96-106 */
+  long t_max_pitches, t_max_strikes;
+  if (pr->max_strikes == 0) { t_max_pitches = 1L << 62; t_max_strikes = 1L << 62; }
+  else { t_max_pitches = pr->max_pitches; t_max_strikes = pr->max_strikes; }
+
+  ml_result *res = malloc(sizeof(ml_result) * (E > 0 ? E : 1));
+  int n_res = 0;
+  double like_record = -INFINITY;
+  long n_strikes = 0;
+  /* play_ball, core.ml:283-314 */
+  for (int rk = 0; rk < E; rk++) {
+    int loc = order[rk];
+    set_tt_edges(&qs, loc, pr->start_pend, c->tree->bl[loc] / 2.);   /* prepare_tt */
+    double ll, pend, dist;
+    long calls = 0;
+    int s2 = safe_ml_optimize(&qs, pr->initial_tolerance, loc, &ll, &pend, &dist, &calls);
+    st_->n_like_calls += calls;
+    if (s2 == OR_EXCEEDED_MAX_ITER) { st_->n_max_iter++; s2 = OR_OK; }
+    if (is_gsl(s2)) { st_->n_skipped++; continue; }   /* skipped with a warning */
+    if (s2 != OR_OK) { st = s2; break; }               /* uncaught: fatal */
+    st_->n_pitched++;
+    int prev_len = n_res;
+    res[n_res].loc = loc; res[n_res].ll = ll; res[n_res].pend = pend; res[n_res].dist = dist;
+    n_res++;
+    if (prev_len >= t_max_pitches) break;
+    else if (ll > like_record) like_record = ll;
+    else if (ll < like_record - pr->strike_box) {
+      if (n_strikes + 1 >= t_max_strikes) break;
+      n_strikes++;
+    }
+  }
+  double t2 = now_s();
+  if (st == OR_OK && n_res == 0) st = OR_NO_RESULTS;
+  if (st == OR_OK) {
+    /* core.ml:336-351: stable sort by likelihood, descending; keep */
+    keyed *kv = malloc(sizeof(keyed) * n_res), *tmp = malloc(sizeof(keyed) * n_res);
+    for (int i = 0; i < n_res; i++) { kv[i].key = res[i].ll; kv[i].idx = i; }
+    merge_sort_desc(kv, tmp, n_res);
+    double best_like = res[kv[0].idx].ll;
+    double log_keep_factor = log(pr->keep_factor);
+    ml_result *kept = malloc(sizeof(ml_result) * n_res);
+    int n_keep = 0;
+    for (int i = 0; i < n_res; i++) {
+      ml_result r = res[kv[i].idx];
+      if (i < pr->keep_at_most && r.ll >= log_keep_factor + best_like) kept[n_keep++] = r;
+    }
+    /* core.ml:354-369: refine at final_tolerance */
+    for (int i = 0; i < n_keep; i++) {
+      ml_result init = kept[i];
+      set_tt_edges(&qs, init.loc, init.pend, init.dist);
+      double ll, pend, dist;
+      long calls = 0;
+      int s2 = safe_ml_optimize(&qs, FINAL_TOLERANCE, init.loc, &ll, &pend, &dist, &calls);
+      st_->n_like_calls += calls;
+      if (s2 == OR_EXCEEDED_MAX_ITER) { st_->n_max_iter++; s2 = OR_OK; }
+      if (s2 == OR_OK) { kept[i].ll = ll; kept[i].pend = pend; kept[i].dist = dist; }
+      else if (!is_gsl(s2)) { st = s2; break; }
+    }
+    if (st == OR_OK) {
+      double *lls = malloc(sizeof(double) * n_keep), *ratio = malloc(sizeof(double) * n_keep);
+      for (int i = 0; i < n_keep; i++) lls[i] = kept[i].ll;
+      or_ll_normalized_prob(lls, n_keep, ratio);
+      /* pquery.ml:86-92: sort placements by ml_ratio descending (stable) */
+      keyed *pk = malloc(sizeof(keyed) * n_keep), *pt = malloc(sizeof(keyed) * n_keep);
+      for (int i = 0; i < n_keep; i++) { pk[i].key = ratio[i]; pk[i].idx = i; }
+      merge_sort_desc(pk, pt, n_keep);
+      for (int i = 0; i < n_keep; i++) {
+        int j = pk[i].idx;
+        out[i].loc = kept[j].loc; out[i].ml_ratio = ratio[j]; out[i].log_like = kept[j].ll;
+        out[i].dist_bl = kept[j].dist; out[i].pend_bl = kept[j].pend;
+      }
+      *n_out = n_keep;
+      free(lls); free(ratio); free(pk); free(pt);
+    }
+    free(kv); free(tmp); free(kept);
+  }
+  st_->t_ml += now_s() - t2 + (t2 - t1);
+  free(res); free(scores); free(order); free(mask);
+  free_tt(&qs.tt);
+  return st;
+}
+
+typedef struct {
+  const or_core *c;
+  const char *queries;
+  int n_queries;
+  atomic_int next;
+  or_placement *out;
+  int *n_out, *status;
+  or_stats *stats;
+  pthread_mutex_t lock;
+} pool_ctx;
+
+static void *pool_worker(void *arg)
+{
+  pool_ctx *p = (pool_ctx *)arg;
+  or_stats local = {0};
+  int K = p->c->prefs.keep_at_most;
+  for (;;) {
+    int q = atomic_fetch_add(&p->next, 1);
+    if (q >= p->n_queries) break;
+    p->status[q] = or_process_query(p->c, p->queries + (long)q * p->c->n_sites,
+                                    p->out + (long)q * K, p->n_out + q, &local);
+  }
+  pthread_mutex_lock(&p->lock);
+  p->stats->n_logdots += local.n_logdots; p->stats->n_like_calls += local.n_like_calls;
+  p->stats->n_pitched += local.n_pitched; p->stats->n_skipped += local.n_skipped;
+  p->stats->n_max_iter += local.n_max_iter;
+  p->stats->t_ranking += local.t_ranking; p->stats->t_ml += local.t_ml;
+  pthread_mutex_unlock(&p->lock);
+  return NULL;
+}
+
+/* The -j N analogue (pplacer_run.ml:628-648): N workers, one query at a time. */
+int or_process_queries(const or_core *c, const char *queries, int n_queries,
+                       int n_threads, or_placement *out, int *n_out, int *status,
+                       or_stats *stats)
+{
+  pool_ctx p;
+  p.c = c; p.queries = queries; p.n_queries = n_queries;
+  atomic_init(&p.next, 0);
+  p.out = out; p.n_out = n_out; p.status = status; p.stats = stats;
+  pthread_mutex_init(&p.lock, NULL);
+  if (n_threads < 1) n_threads = 1;
+  pthread_t *th = malloc(sizeof(pthread_t) * n_threads);
+  for (int i = 0; i < n_threads; i++) pthread_create(&th[i], NULL, pool_worker, &p);
+  for (int i = 0; i < n_threads; i++) pthread_join(th[i], NULL);
+  free(th);
+  pthread_mutex_destroy(&p.lock);
+  for (int q = 0; q < n_queries; q++) if (status[q] != OR_OK) return status[q];
+  return OR_OK;
+}

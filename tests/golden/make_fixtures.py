@@ -1,0 +1,56 @@
+"""Writes the data fixtures under tests/golden/ from the reference checkout.
+
+Run once in a container where /root/reference exists:
+    python tests/golden/make_fixtures.py
+
+* like/{gtr,jtt,wag}/: the alignment, tree and phylo_model files that the
+  reference's tests/pplacer/test_like.ml:50-124 reads (PhyML-verified tree
+  log-likelihoods are recorded in like/expected.json from test_like.ml:105-124).
+* prot_models.json: the WAG, JTT and LG exchangeabilities and frequencies, the
+  numeric tables of pplacer_src/prot_models.ml:70-163, as plain numbers.
+"""
+import json
+import os
+import re
+import shutil
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def copy_like():
+    src = os.path.join(REF, "tests/data/like")
+    files = {
+        "gtr": ["hiv.fasta", "hiv.phy_phyml_tree.txt", "phylo_model.jplace"],
+        "jtt": ["actin.fasta", "actin.phy_phyml_tree.txt", "phylo_model.jplace"],
+        "wag": ["actin.fasta", "actin.phy_phyml_tree.txt", "phylo_model.jplace"],
+    }
+    for d, names in files.items():
+        os.makedirs(os.path.join(HERE, "like", d), exist_ok=True)
+        for n in names:
+            shutil.copy(os.path.join(src, d, n), os.path.join(HERE, "like", d, n))
+    expected = {
+        "gtr": {"fasta": "hiv.fasta", "tree": "hiv.phy_phyml_tree.txt", "log_like": -1752.42340},
+        "jtt": {"fasta": "actin.fasta", "tree": "actin.phy_phyml_tree.txt", "log_like": -2326.22075},
+        "wag": {"fasta": "actin.fasta", "tree": "actin.phy_phyml_tree.txt", "log_like": -2334.79151},
+        "rel_epsilon": 1e-7,
+    }
+    with open(os.path.join(HERE, "like", "expected.json"), "w") as f:
+        json.dump(expected, f, indent=1)
+
+
+def prot_tables():
+    text = open(os.path.join(REF, "pplacer_src/prot_models.ml")).read()
+    out = {}
+    for name in ("wag", "lg", "jtt"):
+        m = re.search(r"let %s_trans =\s*parseProtModel \[\|(.*?)\|\]" % name, text, re.S)
+        rows = [[float(x) for x in row.split()] for row in re.findall(r'"([^"]*)"', m.group(1))]
+        f = re.search(r'let %s_statd = parseFreq "([^"]*)"' % name, text)
+        out[name.upper()] = {"lower": rows, "statd": [float(x) for x in f.group(1).split()]}
+    with open(os.path.join(HERE, "prot_models.json"), "w") as fh:
+        json.dump(out, fh)
+
+
+if __name__ == "__main__":
+    copy_like()
+    prot_tables()
