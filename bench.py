@@ -1,0 +1,193 @@
+#!/usr/bin/env python
+"""Placement throughput on one MI355X.
+
+Workload: the 16S-scale shape -- GTR+Gamma4, 2,000-taxon Yule tree (3,997
+edges, branch lengths ~Exp(0.03)), 1,600 columns, 100,000 queries with
+uniform 100-500-column masked lengths and 2% substitutions -- synthesised
+from a fixed seed.  One step = one full pass of pplacer's per-query placement
+(stage-1 ranking of every (query, edge) pair, baseball + Brent, keep, refine,
+like-weight ratios) over the whole batch, with the queries and the tree's
+GLVs already resident in HBM.
+
+Prints one JSON line.  value = query x edge evaluations per second over the
+whole placement pass (Q * E / step time).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def kernel_work(stats, R=4, S=4, w=8):
+    """Algorithmic flops and bytes of the stage-1 and stage-2 kernels.
+    Stage 1 per (query, edge, masked site): R*S products and sums (2*R*S
+    flops) and the supernode row (R*S*w bytes + 4-byte exponent).
+    Stage 2 per probe and masked site: pendant probe 2*R*S flops (util);
+    distal probe adds two R*S*S evolves (4*R*S*S) and the pi*x*y products
+    (2*R*S)."""
+    s1_flops = 2 * R * S * stats.n_site_pairs
+    s1_bytes = (R * S * w + 4) * stats.n_site_pairs
+    s2_flops = (2 * R * S) * stats.n_site_probes_pend + (4 * R * S * S + 4 * R * S) * stats.n_site_probes_dist
+    return s1_flops, s1_bytes, s2_flops
+
+
+def cpu_baseline(wl, queries, n_sample, threads):
+    """The oracle (FP64 restatement of pplacer, one query per worker like a
+    pplacer -j child) on a bounded sample of the same workload."""
+    from oracle import pyoracle as po
+    S = wl.n_states
+    ref_seqs = [s for _, s in wl.ref_aln]
+    model = po.Model(wl.exch, po.emper_freq(S, ref_seqs), po.discrete_gamma(wl.n_cats, wl.alpha))
+    st, mask, keep, n = po.premask(S, ref_seqs, [s for _, s in wl.queries])
+    import numpy as np
+    m = np.asarray(mask, dtype=bool)
+    leaf = {i: np.frombuffer(s.encode(), np.uint8)[m].tobytes().decode()
+            for i, s in wl.leaf_seqs().items()}
+    pre = po.Precompute(model, wl.tree, leaf, n)
+    pre.pull_and_supernodes()
+    core = po.Core(pre)
+    sample = queries[:n_sample]
+    t0 = time.perf_counter()
+    st, res, status, stats = core.place_many(sample, n_threads=threads)
+    dt = time.perf_counter() - t0
+    E = wl.tree.n_nodes - 1
+    return {"value": len(sample) * E / dt, "unit": "query x edge evals/s", "cores": threads,
+            "kind": "port",
+            "sample": "%d queries of the same workload (FP64 oracle, %d threads, %.1f s wall, "
+                      "%.1f ms/query/thread)" % (len(sample), threads, dt,
+                                                  1000 * dt * threads / len(sample)),
+            "queries_per_s": len(sample) / dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--queries", type=int, default=100_000)
+    ap.add_argument("--workload", default="gtr2000")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--other-form", action=argparse.BooleanOptionalAction, default=True,
+                    help="also time the other precision form (reported under 'forms')")
+    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(queries would shard across GPUs with replicated tree GLVs).", file=sys.stderr)
+        sys.exit(2)
+
+    from pplacer_amd import placer, synth
+    from pplacer_amd._lib import PP_FP32, PP_FP64
+
+    log("synthesising workload %s (%d queries, seed %d)" % (args.workload, args.queries, args.seed))
+    t0 = time.perf_counter()
+    wl = synth.make_workload(args.workload, seed=args.seed, n_queries=args.queries)
+    log("synthesised in %.1f s; building model, pre-mask and tree GLVs" % (time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    pl, queries, statd, rates = placer.setup_from_workload(wl)
+    E = wl.tree.n_nodes - 1
+    N = len(queries[0])
+    log("tree ready in %.1f s (G=%d, N=%d); uploading queries" % (time.perf_counter() - t0,
+                                                                  wl.tree.n_nodes, N))
+    pl.upload_queries(queries)
+    prefs = placer.default_prefs()
+
+    def run_form(prec_name):
+        prec = PP_FP64 if prec_name == "fp64" else PP_FP32
+        for i in range(args.warmup):
+            log("%s warmup %d" % (prec_name, i))
+            pl.place_resident(prefs, prec)
+        pl.synchronize()
+        times, stats_l = [], []
+        for i in range(args.steps):
+            pl.synchronize()
+            t0 = time.perf_counter()
+            st = pl.place_resident(prefs, prec)
+            pl.synchronize()
+            times.append(time.perf_counter() - t0)
+            stats_l.append(st)
+            log("%s step %d: %.3f s (stage1 %.1f ms, pitch %.1f ms, refine %.1f ms)" % (
+                prec_name, i, times[-1], st.t_stage1_ms, st.t_pitch_ms, st.t_refine_ms))
+        return times, stats_l
+
+    forms = {}
+    order = [args.precision] + ([("fp32" if args.precision == "fp64" else "fp64")] if args.other_form else [])
+    for f in order:
+        times, stats_l = run_form(f)
+        st = stats_l[-1]
+        ms = 1000.0 * sum(times) / len(times)
+        w = 8 if f == "fp64" else 4
+        s1_flops, s1_bytes, s2_flops = kernel_work(st, w=w)
+        forms[f] = {
+            "ms_per_step": ms,
+            "value": st.n_logdots / (ms / 1000.0),
+            "queries_per_s": args.queries / (ms / 1000.0),
+            "t_stage1_ms": st.t_stage1_ms, "t_pitch_ms": st.t_pitch_ms,
+            "t_refine_ms": st.t_refine_ms,
+            "stage1_tflops": s1_flops / (st.t_stage1_ms * 1e-3) / 1e12,
+            "stage1_untiled_GBps": s1_bytes / (st.t_stage1_ms * 1e-3) / 1e9,
+            "stage2_tflops": s2_flops / ((st.t_pitch_ms + st.t_refine_ms) * 1e-3) / 1e12,
+            "n_logdots": st.n_logdots, "n_like_calls": st.n_like_calls,
+            "n_pitched": st.n_pitched, "n_launches": st.n_launches,
+        }
+    main_f = forms[args.precision]
+    st_main = main_f
+    # dominant kernel: the one with the largest device time
+    kern = max([("stage1_kernel", st_main["t_stage1_ms"]), ("pitch_kernel", st_main["t_pitch_ms"]),
+                ("refine_kernels", st_main["t_refine_ms"])], key=lambda kv: kv[1])[0]
+    peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
+    if kern == "stage1_kernel":
+        achieved = st_main["stage1_tflops"]
+    else:
+        achieved = st_main["stage2_tflops"]
+    roofline = {"bound": "mfma", "pipe": "vector ALU %s (4-state contractions do not use MFMA)" % args.precision,
+                "kernel": kern, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None}
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log("CPU baseline: %d queries on %d threads" % (args.cpu_sample, threads))
+        cpu = cpu_baseline(wl, queries, args.cpu_sample, threads)
+    out = {
+        "metric": "placements/sec (query x edge evals/s), full placement pass",
+        "value": main_f["value"],
+        "unit": "query x edge evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": main_f["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic (seeded simulation)",
+        "config": {"workload": "16S-scale GTR+G4: 2000-taxon Yule tree (3997 edges), 1600 columns, "
+                               "%d queries, masked length 100-500, 2%% substitutions" % args.queries,
+                   "queries": args.queries, "edges": E, "columns": N,
+                   "form": "plain FP64" if args.precision == "fp64" else "batched FP32"},
+        "queries_per_s": main_f["queries_per_s"],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "forms": forms,
+    }
+    if cpu:
+        out["gpu_vs_cpu"] = main_f["value"] / cpu["value"]
+    print(json.dumps(out), flush=True)
+    pl.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,223 @@
+/*
+ * pplacer_hip.h -- C ABI of libpplacer_hip.so, the MI355X (gfx950) placement
+ * path of pplacer.  Plain pointers and sizes only; every entry point returns 0
+ * on success or a PP_E* code, and pp_last_error() gives the thread's message.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Per-call twins of the C stubs that pplacer_src/linear.ml:13-89 binds
+ *     (`external ... = "ten_masked_logdot_c"` etc.).  Same arguments, same
+ *     arithmetic, host buffers in and out; the work runs on the GPU.  A
+ *     maintainer re-points each `external` at a thin OCaml stub that unpacks
+ *     the Bigarrays and calls the twin (INTEGRATION.md has the stub file).
+ *
+ *  2. A batched placement API replacing the per-query loop of
+ *     Core.pplacer_core (pplacer_src/core.ml:76-411) as driven by
+ *     Pplacer_run.run_placements (pplacer_src/pplacer_run.ml:167-656):
+ *     model build, reference-tree precompute, stage-1 ranking of every
+ *     (query, edge) pair, stage-2 baseball + Brent branch-length
+ *     optimisation, keep/refine and like-weight ratios, all device-resident.
+ *
+ * Layout conventions follow pplacer's GLVs (pplacer_src/linear_c.c:5-12): a
+ * GLV is double a[rate][site][state] (c_layout) plus one exponent per site.
+ */
+#ifndef PPLACER_HIP_H
+#define PPLACER_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PP_OK = 0,
+  PP_EINVAL = 1,     /* bad arguments or dimensions */
+  PP_EDEVICE = 2,    /* HIP runtime error */
+  PP_ENOMEM = 3,     /* device allocation failed */
+  PP_ESTATE = 4,     /* call out of order (e.g. no model / tree yet) */
+  PP_EQUERY = 5      /* a query failed the way core.ml fails it (see status) */
+};
+
+/* per-query status of the batched API (mirrors the exceptions core.ml lets
+ * escape from process_query) */
+enum {
+  PP_Q_OK = 0,
+  PP_Q_NO_INFORMATIVE = 1, /* "sequence has no informative sites" core.ml:157-158 */
+  PP_Q_BAD_CHAR = 2,       /* "not a known base" nuc_models.ml:49-54 */
+  PP_Q_NO_RESULTS = 3,     /* "empty results" core.ml:319-321 */
+  PP_Q_FIND_START = 4,     /* Minimization.FindStartFailure */
+  PP_Q_INVALID_START = 5,  /* Minimization.InvalidStartValues */
+  PP_Q_ASSERT = 6,         /* assert(prox_bl >= 0.) three_tax.ml:43 */
+  PP_Q_RANK_EXHAUSTED = 7  /* internal: ranking window too short (host retries) */
+};
+
+enum { PP_FP64 = 0, PP_FP32 = 1 };
+
+const char *pp_last_error(void);
+
+/* ======================================================================= */
+/* 1. linear_c.c twins (host buffers; GPU compute)                          */
+/* ======================================================================= */
+
+/* linear_c.c:29-90 gemmish_c: dst[site][i] = sum_j a[i][j] * b[site][j] */
+int pp_gemmish(double *dst, const double *a, const double *b, int n_sites, int n_states);
+
+/* linear_c.c:92-146 dediagonalize: dst[i][j] = sum_k lambda[k] u[i][k] uit[j][k] */
+int pp_dediagonalize(double *dst, const double *u, const double *lambda, const double *uit,
+                     int n);
+
+/* linear_c.c:367-424 ten_log_like3_c -> *out; util[n_sites] receives the
+ * per-site sums as the stub leaves them */
+int pp_ten_log_like3(const double *statd, const double *x, const double *y, const double *z,
+                     double *util, int n_rates, int n_sites, int n_states, double *out);
+
+/* linear_c.c:426-441 ten_pairwise_prod_c */
+int pp_ten_pairwise_prod(double *dst, const double *x, const double *y, int n_rates,
+                         int n_sites, int n_states);
+
+/* linear_c.c:443-463 ten_statd_pairwise_prod_c */
+int pp_ten_statd_pairwise_prod(const double *statd, double *dst, const double *a,
+                               const double *b, int n_rates, int n_sites, int n_states);
+
+/* linear_c.c:465-539 ten_masked_logdot_c -> *out */
+int pp_ten_masked_logdot(const double *x, const double *y, const uint16_t *mask, double *util,
+                         int n_rates, int n_sites, int n_states, double *out);
+
+/* linear_c.c:541-605 ten_bounded_logdot_c -> *out */
+int pp_ten_bounded_logdot(const double *x, const double *y, int first, int last, double *util,
+                          int n_rates, int n_sites, int n_states, double *out);
+
+/* ======================================================================= */
+/* 2. batched placement                                                     */
+/* ======================================================================= */
+
+typedef struct pp_ctx pp_ctx;
+
+/* One context owns one GPU's resident model, tree GLVs and work buffers.
+ * Create it after any fork() (pplacer forks children, pplacer_run.ml:628). */
+int pp_ctx_create(int device, pp_ctx **out);
+int pp_ctx_destroy(pp_ctx *ctx);
+
+/* Glvm.Model build for the gamma mixture model (gmix_model.ml:25-37):
+ * exch = symmetric exchangeabilities [S][S] (diagonal ignored), statd [S],
+ * rates [R] (Gamma.discrete_gamma).  Diagonalises Q = B D in FP64
+ * (diagd.ml:123-172). */
+int pp_model_build(pp_ctx *ctx, int n_states, int n_rates, const double *exch,
+                   const double *statd, const double *rates);
+/* read back the diagonalisation: x [S][S], l [S], xit [S][S] */
+int pp_model_get(pp_ctx *ctx, double *x, double *l, double *xit);
+
+/* Gamma.discrete_gamma (gamma.ml:52-70) */
+int pp_discrete_gamma(int n_cat, double alpha, double *rates);
+
+/* Alignment.emper_freq (alignment.ml:186-217) over [n_seq][n_sites] chars */
+int pp_emper_freq(int n_states, const char *seqs, int n_seq, int n_sites, double *freq);
+
+/* Pplacer_run.premask (pplacer_run.ml:59-144): mask[N0] of columns kept,
+ * query_keep[Q], *masklen */
+int pp_premask(int n_states, const char *ref_seqs, int n_ref, const char *query_seqs,
+               int n_query, int n_sites, int discard_nonoverlapped, uint8_t *mask,
+               uint8_t *query_keep, int *masklen);
+
+/* Reference tree + leaf sequences (already pre-masked, n_sites columns).
+ * Node ids are pplacer's post-order ids (newick_parser.mly:49-62); root is
+ * n_nodes-1 with bl 0.  child_start[n_nodes+1]/children[] list each node's
+ * children in Newick order; leaf_seqs is [n_nodes][n_sites] (rows of internal
+ * nodes ignored).  Runs Like_stree.calc_distal_and_proximal, the -10
+ * exponent pull and Glv_arr.prep_supernodes (pplacer_run.ml:321-341) on the
+ * GPU and keeps darr/parr/snodes resident. */
+int pp_tree_set(pp_ctx *ctx, int n_nodes, const int *child_start, const int *children,
+                const double *bl, int n_sites, const char *leaf_seqs);
+
+/* Alternative to pp_tree_set for a host that computed the GLVs itself (the
+ * OCaml host's darr/parr/snodes after prep_supernodes): arrays are
+ * [G][R][N][S] doubles and [G][N] int64 exponents. */
+int pp_tree_upload(pp_ctx *ctx, int n_nodes, const double *bl, int n_sites,
+                   const double *darr, const int64_t *darr_e, const double *parr,
+                   const int64_t *parr_e, const double *snodes, const int64_t *snodes_e);
+
+/* Copy the resident GLVs back in pplacer's layout ([G][R][N][S], [G][N]). */
+int pp_tree_download(pp_ctx *ctx, double *darr, int64_t *darr_e, double *parr,
+                     int64_t *parr_e, double *snodes, int64_t *snodes_e);
+
+/* Prefs that steer Core.pplacer_core (prefs.ml:65-121 defaults in brackets). */
+typedef struct {
+  double start_pend;        /* 0.1  */
+  double max_pend;          /* 2.0  */
+  double initial_tolerance; /* 0.01 */
+  double strike_box;        /* 3.0  */
+  double keep_factor;       /* 0.01 */
+  int max_strikes;          /* 6 (0 = evaluate every edge) */
+  int max_pitches;          /* 40 */
+  int keep_at_most;         /* 7 */
+} pp_prefs;
+
+void pp_prefs_default(pp_prefs *p);
+
+/* Placement.make_ml fields (placement.ml:50-60) */
+typedef struct {
+  int loc;          /* edge_num */
+  double ml_ratio;  /* like_weight_ratio */
+  double log_like;  /* likelihood */
+  double dist_bl;   /* distal_length */
+  double pend_bl;   /* pendant_length */
+} pp_placement;
+
+typedef struct {
+  double t_ranking_ms;   /* stage 1 + rank ("ranking" timing, core.ml:212) */
+  double t_ml_ms;        /* stage 2 ("ML calculation", core.ml:322) */
+  int64_t n_logdots;     /* (query, edge) stage-1 scores */
+  int64_t n_like_calls;  /* log_like3 evaluations (Three_tax.n_like_calls) */
+  int64_t n_pitched;     /* Brent-optimised (query, edge) pairs */
+  /* per-kernel device time (HIP events on the library's stream) */
+  double t_stage1_ms;    /* stage-1 scoring kernel */
+  double t_pitch_ms;     /* baseball + Brent kernels */
+  double t_refine_ms;    /* keep + final refinement + finalize kernels */
+  /* algorithmic work: sums over evaluations of the masked length M */
+  int64_t n_site_pairs;       /* sum over (query, edge) of M  (stage 1) */
+  int64_t n_site_probes_pend; /* sum over pendant-only probes of M (stage 2) */
+  int64_t n_site_probes_dist; /* sum over probes that re-evolve prox/dist of M */
+  int64_t n_launches;         /* kernel launches issued */
+} pp_stats;
+
+/* Stage 1 for a batch: scores[q][e] = Glv.masked_logdot of the query (full
+ * length, evolved by start_pend) against supernode e, for every non-root
+ * edge e in 0..n_nodes-2 (core.ml:131-141,187-194).  queries is
+ * [n_queries][n_sites] characters (upper case). */
+int pp_stage1_scores(pp_ctx *ctx, int n_queries, const char *queries, double start_pend,
+                     int precision, double *scores);
+
+/* Stage-1 ranking: for each query the first k edges of the stable
+ * descending sort of its scores (fig.ml:143-147; nan last, ties by edge id). */
+int pp_rank_topk(pp_ctx *ctx, int n_queries, int n_edges, const double *scores, int k,
+                 int *ranked);
+
+/* Three_tax.optimize from a given start (core.ml:225-281): one job per
+ * (query, loc, pend0, distal0); results are the last-probe state
+ * (ll, pend, distal) as Three_tax.get_results reports it. */
+int pp_optimize_batch(pp_ctx *ctx, int n_jobs, const char *queries, const int *query_of_job,
+                      const int *locs, const double *pend0, const double *dist0,
+                      double tolerance, double max_pend, int precision, double *ll,
+                      double *pend, double *dist, int *n_calls, int *status);
+
+/* Full Core.process_query for a batch of queries: stage 1, ranking,
+ * baseball with Brent at initial_tolerance, keep, refine at 1e-5,
+ * ll_normalized_prob and the ml_ratio sort.  out is [n_queries][keep_at_most],
+ * n_out[q] placements each; status[q] is a PP_Q_* code. */
+int pp_place_batch(pp_ctx *ctx, int n_queries, const char *queries, const pp_prefs *prefs,
+                   int precision, pp_placement *out, int *n_out, int *status,
+                   pp_stats *stats);
+
+/* Device-resident variant for throughput measurement and pipelines: the
+ * queries are uploaded once (pp_queries_upload), placed with no host
+ * transfer inside pp_place_resident, and fetched with pp_results_download. */
+int pp_queries_upload(pp_ctx *ctx, int n_queries, const char *queries);
+int pp_place_resident(pp_ctx *ctx, const pp_prefs *prefs, int precision, pp_stats *stats);
+int pp_results_download(pp_ctx *ctx, pp_placement *out, int *n_out, int *status);
+int pp_synchronize(pp_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,984 @@
+// C ABI of libpplacer_hip.so (include/pplacer_hip.h): device context, data
+// layout in HBM, and the launch sequence of the batched placement path.
+//
+// HBM layout of a resident tree (G nodes, N sites, E = G-1 edges, R rates):
+//   darr, parr : [G][N][R*4] FP64 and FP32 copies, exponents int32 [G][N]
+//                (a masked site of one edge is one contiguous row: stage 2
+//                 gathers rows of the query's informative sites)
+//   snodes     : [N][Epad][R*4] FP64 and FP32 (site-major: stage 1 lanes =
+//                 edges read consecutive rows), exponents int32 [N][Epad]
+// Queries are stored as 1-byte character codes [Q][N], sorted by first
+// informative column; results are un-permuted on download.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "pp_kernels.hip"
+#include "pp_tree.h"
+
+using namespace pp;
+
+#define HIPCHECK(expr)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      set_error(std::string(#expr) + " failed: " + hipGetErrorString(e_));          \
+      return PP_EDEVICE;                                                            \
+    }                                                                               \
+  } while (0)
+
+#define CHECK(cond, code, msg) \
+  do {                         \
+    if (!(cond)) {             \
+      set_error(msg);          \
+      return code;             \
+    }                          \
+  } while (0)
+
+namespace {
+
+constexpr int kTQ = 16;           // queries per stage-1 tile
+constexpr int kChunk = 32768;     // queries per device batch
+
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+template <class T>
+int ensure(DBuf &b, size_t count, T **out) {
+  size_t need = std::max<size_t>(count * sizeof(T), 64);
+  if (b.bytes < need) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    if (hipMalloc(&b.p, need) != hipSuccess) {
+      set_error("hipMalloc of " + std::to_string(need) + " bytes failed");
+      return PP_ENOMEM;
+    }
+    b.bytes = need;
+  }
+  *out = reinterpret_cast<T *>(b.p);
+  return PP_OK;
+}
+
+#define ENSURE(buf, n, ptr)                  \
+  do {                                       \
+    int st_ = ensure(buf, (size_t)(n), ptr); \
+    if (st_) return st_;                     \
+  } while (0)
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct pp_ctx {
+  int device = 0;
+  int n_cu = 256;
+  hipStream_t stream = nullptr;
+  HostModel model;
+  DBuf b_model;
+  DevModel *d_model = nullptr;
+  // tree
+  bool has_tree = false;
+  int G = 0, N = 0, E = 0, Epad = 0;
+  std::vector<double> bl;
+  DBuf b_bl, b_d64, b_p64, b_d32, b_p32, b_de, b_pe, b_s64, b_s32, b_se;
+  // resident queries (sorted by first informative column)
+  int Q = 0;
+  std::vector<int> order;     // sorted position -> caller's index
+  std::vector<int> qM, qlo, qhi, qstat;
+  DBuf b_codes, b_qM;
+  // work buffers
+  DBuf b_tcodes, b_tile_lo, b_tile_hi, b_tile_q, b_ttab, b_scores, b_res, b_nres, b_status,
+      b_kept, b_nkeep, b_out, b_nout, b_qlist, b_counter, b_stats;
+  // results staging (sorted order)
+  std::vector<pp_placement> h_out;
+  std::vector<int> h_nout, h_status;
+  int K_last = 0;
+};
+
+namespace {
+
+int set_device(pp_ctx *c) {
+  HIPCHECK(hipSetDevice(c->device));
+  return PP_OK;
+}
+
+// stage-1 table: every character's LV evolved by start_pend (the rows of
+// full_query_evolv, core.ml:137-141), [n_codes][R*S], FP64 as the stubs do
+std::vector<double> query_table(const HostModel &m, double bl) {
+  const int R = m.R, S = m.S;
+  std::vector<double> P(R * S * S), T(m.n_codes * R * S);
+  multi_exp(m, bl, P.data());
+  for (int c = 0; c < m.n_codes; c++)
+    for (int r = 0; r < R; r++)
+      for (int i = 0; i < S; i++) {
+        double acc = 0;
+        for (int j = 0; j < S; j++) acc += P[r * S * S + i * S + j] * m.lv[c * S + j];
+        T[(c * R + r) * S + i] = acc;
+      }
+  return T;
+}
+
+template <typename Real>
+void to_device_layout(const TreeGlvs &g, int G, int N, int R, int S, int Epad,
+                      std::vector<Real> &darr, std::vector<Real> &parr, std::vector<Real> &sn) {
+  const int RS = R * S;
+  darr.assign((size_t)G * N * RS, 0);
+  parr.assign((size_t)G * N * RS, 0);
+  sn.assign((size_t)N * Epad * RS, 0);
+  for (int i = 0; i < G; i++)
+    for (int r = 0; r < R; r++)
+      for (int n = 0; n < N; n++)
+        for (int s = 0; s < S; s++) {
+          const size_t src = (((size_t)i * R + r) * N + n) * S + s;
+          const size_t dst = ((size_t)i * N + n) * RS + r * S + s;
+          darr[dst] = (Real)g.darr[src];
+          parr[dst] = (Real)g.parr[src];
+          if (i < G - 1) sn[((size_t)n * Epad + i) * RS + r * S + s] = (Real)g.snodes[src];
+        }
+}
+
+int upload_glvs(pp_ctx *c, const TreeGlvs &g) {
+  const int G = c->G, N = c->N, R = c->model.R, S = c->model.S, Epad = c->Epad;
+  if (S != 4) { set_error("the GPU placement path supports nucleotide models (n_states = 4)"); return PP_EINVAL; }
+  const size_t glv = (size_t)G * N * R * S, snn = (size_t)N * Epad * R * S;
+  double *d64, *p64, *s64;
+  float *d32, *p32, *s32;
+  int *de, *pe, *se;
+  ENSURE(c->b_d64, glv, &d64);
+  ENSURE(c->b_p64, glv, &p64);
+  ENSURE(c->b_s64, snn, &s64);
+  ENSURE(c->b_d32, glv, &d32);
+  ENSURE(c->b_p32, glv, &p32);
+  ENSURE(c->b_s32, snn, &s32);
+  ENSURE(c->b_de, (size_t)G * N, &de);
+  ENSURE(c->b_pe, (size_t)G * N, &pe);
+  ENSURE(c->b_se, (size_t)N * Epad, &se);
+  {
+    std::vector<double> a, b, s;
+    to_device_layout<double>(g, G, N, R, S, Epad, a, b, s);
+    HIPCHECK(hipMemcpy(d64, a.data(), a.size() * 8, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(p64, b.data(), b.size() * 8, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(s64, s.data(), s.size() * 8, hipMemcpyHostToDevice));
+  }
+  {
+    std::vector<float> a, b, s;
+    to_device_layout<float>(g, G, N, R, S, Epad, a, b, s);
+    HIPCHECK(hipMemcpy(d32, a.data(), a.size() * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(p32, b.data(), b.size() * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(s32, s.data(), s.size() * 4, hipMemcpyHostToDevice));
+  }
+  std::vector<int> ed((size_t)G * N), ep((size_t)G * N), es((size_t)N * Epad, 0);
+  for (size_t i = 0; i < ed.size(); i++) { ed[i] = (int)g.darr_e[i]; ep[i] = (int)g.parr_e[i]; }
+  for (int i = 0; i < G - 1; i++)
+    for (int n = 0; n < N; n++) es[(size_t)n * Epad + i] = (int)g.snodes_e[(size_t)i * N + n];
+  HIPCHECK(hipMemcpy(de, ed.data(), ed.size() * 4, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(pe, ep.data(), ep.size() * 4, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(se, es.data(), es.size() * 4, hipMemcpyHostToDevice));
+  double *dbl;
+  ENSURE(c->b_bl, G, &dbl);
+  HIPCHECK(hipMemcpy(dbl, c->bl.data(), G * 8, hipMemcpyHostToDevice));
+  c->has_tree = true;
+  return PP_OK;
+}
+
+int bucket_of(int M) { return M <= 0 ? 0 : (M - 1) / 64; }  // block = 64*(bucket+1)
+
+template <typename Real>
+int launch_stage1(pp_ctx *c, int q0, int nq, double start_pend, double *scores,
+                  hipEvent_t ev_start = nullptr) {
+  const int N = c->N, R = c->model.R;
+  const int n_tiles = (nq + kTQ - 1) / kTQ;
+  std::vector<int> tlo(n_tiles), thi(n_tiles), tq(n_tiles * kTQ, -1);
+  for (int t = 0; t < n_tiles; t++) {
+    int lo = N, hi = 0;
+    for (int j = 0; j < kTQ; j++) {
+      int l = t * kTQ + j;
+      if (l >= nq) break;
+      int q = q0 + l;
+      if (c->qstat[q] != PP_Q_OK) continue;
+      tq[t * kTQ + j] = l;
+      lo = std::min(lo, c->qlo[q]);
+      hi = std::max(hi, c->qhi[q]);
+    }
+    if (hi < lo) { lo = 0; hi = 0; }
+    tlo[t] = lo;
+    thi[t] = hi;
+  }
+  int *d_tlo, *d_thi, *d_tq;
+  uint8_t *d_tc;
+  double *d_ttab;
+  ENSURE(c->b_tile_lo, n_tiles, &d_tlo);
+  ENSURE(c->b_tile_hi, n_tiles, &d_thi);
+  ENSURE(c->b_tile_q, n_tiles * kTQ, &d_tq);
+  ENSURE(c->b_tcodes, (size_t)n_tiles * N * kTQ, &d_tc);
+  std::vector<double> T = query_table(c->model, start_pend);
+  ENSURE(c->b_ttab, T.size(), &d_ttab);
+  HIPCHECK(hipMemcpyAsync(d_tlo, tlo.data(), n_tiles * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(d_thi, thi.data(), n_tiles * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(d_tq, tq.data(), n_tiles * kTQ * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(d_ttab, T.data(), T.size() * 8, hipMemcpyHostToDevice, c->stream));
+  const uint8_t *codes = reinterpret_cast<const uint8_t *>(c->b_codes.p) + (size_t)q0 * N;
+  hipLaunchKernelGGL(tile_codes_kernel, dim3(2048), dim3(256), 0, c->stream, codes, d_tq, n_tiles,
+                     N, kTQ, (uint8_t)c->model.n_inf, d_tc);
+  Stage1Args a;
+  a.sn = std::is_same<Real, double>::value ? c->b_s64.p : c->b_s32.p;
+  a.N = N;
+  a.E = c->E;
+  a.Epad = c->Epad;
+  a.n_codes = c->model.n_codes;
+  a.n_inf = c->model.n_inf;
+  a.sn_e = reinterpret_cast<const int *>(c->b_se.p);
+  a.tcodes = d_tc;
+  a.tile_lo = d_tlo;
+  a.tile_hi = d_thi;
+  a.tile_q = d_tq;
+  a.qM = reinterpret_cast<const int *>(c->b_qM.p) + q0;
+  a.ttab = d_ttab;
+  a.log_R = std::log((double)(float)R);
+  a.scores = scores;
+  dim3 grid(c->Epad / 256, n_tiles);
+  if (ev_start) HIPCHECK(hipEventRecord(ev_start, c->stream));
+  if (R == 4) hipLaunchKernelGGL((stage1_kernel<Real, 4, kTQ>), grid, dim3(256), 0, c->stream, a);
+  else if (R == 1) hipLaunchKernelGGL((stage1_kernel<Real, 1, kTQ>), grid, dim3(256), 0, c->stream, a);
+  else { set_error("n_rates must be 1 or 4 on the GPU path"); return PP_EINVAL; }
+  HIPCHECK(hipGetLastError());
+  return PP_OK;
+}
+
+template <typename Real, int R>
+int occupancy_grid(const void *kern, int bd, size_t lds, int n_cu, int n_items) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, bd, lds) != hipSuccess || nb < 1) nb = 1;
+  return std::max(1, std::min(n_items, nb * n_cu));
+}
+
+// Blocks of up to 512 lanes get a 256-VGPR budget; longer queries (M > 512)
+// use the 1024-lane instantiation.
+template <typename Real, int R>
+int launch_pitch(pp_ctx *c, PitchArgs a, int bd) {
+  const size_t lds = Engine4<Real, R>::engine_lds_bytes(c->N) + 4 * ((c->E + 31) / 32) + 64;
+  const void *k = bd <= 512 ? reinterpret_cast<const void *>(&pitch_kernel<Real, R, 512>)
+                            : reinterpret_cast<const void *>(&pitch_kernel<Real, R, 1024>);
+  int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.nq);
+  if (bd <= 512) hipLaunchKernelGGL((pitch_kernel<Real, R, 512>), dim3(grid), dim3(bd), lds, c->stream, a);
+  else hipLaunchKernelGGL((pitch_kernel<Real, R, 1024>), dim3(grid), dim3(bd), lds, c->stream, a);
+  HIPCHECK(hipGetLastError());
+  return PP_OK;
+}
+
+template <typename Real, int R>
+int launch_opt(pp_ctx *c, OptArgs a, int bd) {
+  const size_t lds = Engine4<Real, R>::engine_lds_bytes(c->N) + 64;
+  const void *k = bd <= 512 ? reinterpret_cast<const void *>(&optimize_kernel<Real, R, 512>)
+                            : reinterpret_cast<const void *>(&optimize_kernel<Real, R, 1024>);
+  int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.n_jobs);
+  if (bd <= 512) hipLaunchKernelGGL((optimize_kernel<Real, R, 512>), dim3(grid), dim3(bd), lds, c->stream, a);
+  else hipLaunchKernelGGL((optimize_kernel<Real, R, 1024>), dim3(grid), dim3(bd), lds, c->stream, a);
+  HIPCHECK(hipGetLastError());
+  return PP_OK;
+}
+
+template <typename Real>
+int dispatch_pitch(pp_ctx *c, PitchArgs a, int bd) {
+  if (c->model.R == 4) return launch_pitch<Real, 4>(c, a, bd);
+  if (c->model.R == 1) return launch_pitch<Real, 1>(c, a, bd);
+  set_error("n_rates must be 1 or 4 on the GPU path");
+  return PP_EINVAL;
+}
+template <typename Real>
+int dispatch_opt(pp_ctx *c, OptArgs a, int bd) {
+  if (c->model.R == 4) return launch_opt<Real, 4>(c, a, bd);
+  if (c->model.R == 1) return launch_opt<Real, 1>(c, a, bd);
+  set_error("n_rates must be 1 or 4 on the GPU path");
+  return PP_EINVAL;
+}
+
+// Encode and upload queries (sorted by first informative column).
+int queries_upload(pp_ctx *c, int n_queries, const char *queries) {
+  CHECK(c->model.built, PP_ESTATE, "no model: call pp_model_build first");
+  CHECK(c->has_tree, PP_ESTATE, "no tree: call pp_tree_set or pp_tree_upload first");
+  CHECK(n_queries >= 0, PP_EINVAL, "n_queries < 0");
+  const int N = c->N;
+  c->Q = n_queries;
+  std::vector<int> lo(n_queries), hi(n_queries), M(n_queries), st(n_queries, PP_Q_OK);
+  std::vector<uint8_t> codes((size_t)n_queries * N);
+  const int n_inf = c->model.n_inf;
+  for (int q = 0; q < n_queries; q++) {
+    const char *s = queries + (size_t)q * N;
+    uint8_t *o = codes.data() + (size_t)q * N;
+    int l = N, h = 0, m = 0;
+    for (int k = 0; k < N; k++) {
+      int code = c->model.code_of[(unsigned char)s[k]];
+      if (code < 0) { st[q] = PP_Q_BAD_CHAR; code = n_inf; }
+      o[k] = (uint8_t)code;
+      if (code < n_inf) { m++; l = std::min(l, k); h = k + 1; }
+    }
+    if (st[q] == PP_Q_OK && m == 0) st[q] = PP_Q_NO_INFORMATIVE;
+    if (st[q] == PP_Q_OK && m > 1024) st[q] = PP_Q_RANK_EXHAUSTED;
+    lo[q] = m ? l : 0;
+    hi[q] = m ? h : 0;
+    M[q] = m;
+  }
+  c->order.resize(n_queries);
+  std::iota(c->order.begin(), c->order.end(), 0);
+  std::stable_sort(c->order.begin(), c->order.end(), [&](int a, int b) { return lo[a] < lo[b]; });
+  c->qM.resize(n_queries);
+  c->qlo.resize(n_queries);
+  c->qhi.resize(n_queries);
+  c->qstat.resize(n_queries);
+  std::vector<uint8_t> sorted((size_t)n_queries * N);
+  for (int i = 0; i < n_queries; i++) {
+    int q = c->order[i];
+    c->qM[i] = M[q];
+    c->qlo[i] = lo[q];
+    c->qhi[i] = hi[q];
+    c->qstat[i] = st[q];
+    std::memcpy(sorted.data() + (size_t)i * N, codes.data() + (size_t)q * N, N);
+  }
+  uint8_t *d_codes;
+  int *d_qM;
+  ENSURE(c->b_codes, (size_t)n_queries * N, &d_codes);
+  ENSURE(c->b_qM, n_queries, &d_qM);
+  HIPCHECK(hipMemcpy(d_codes, sorted.data(), sorted.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_qM, c->qM.data(), (size_t)n_queries * 4, hipMemcpyHostToDevice));
+  return PP_OK;
+}
+
+int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats) {
+  CHECK(prefs, PP_EINVAL, "prefs is null");
+  CHECK(precision == PP_FP64 || precision == PP_FP32, PP_EINVAL, "precision must be PP_FP64 or PP_FP32");
+  CHECK(prefs->keep_at_most >= 0 && prefs->keep_at_most <= 64, PP_EINVAL, "keep_at_most must be in 0..64");
+  int st = set_device(c);
+  if (st) return st;
+  const int Q = c->Q, E = c->E, N = c->N, K = std::max(1, prefs->keep_at_most);
+  c->K_last = K;
+  long long t_max_pitches, t_max_strikes;
+  if (prefs->max_strikes == 0) { t_max_pitches = 1LL << 62; t_max_strikes = 1LL << 62; }
+  else { t_max_pitches = prefs->max_pitches; t_max_strikes = prefs->max_strikes; }
+  const int cap = (int)std::min<long long>(E, std::max<long long>(1, t_max_pitches + 1));
+  const int chunk = std::max(1, std::min(Q, (int)std::min<long long>(kChunk, (1LL << 31) / ((long long)E * 8 + 1))));
+  pp_placement *d_out;
+  int *d_nout, *d_status;
+  ENSURE(c->b_out, (size_t)std::max(Q, 1) * K, &d_out);
+  ENSURE(c->b_nout, std::max(Q, 1), &d_nout);
+  ENSURE(c->b_status, std::max(Q, 1), &d_status);
+  double *d_scores;
+  Res *d_res, *d_kept;
+  int *d_nres, *d_nkeep, *d_qlist, *d_counter;
+  unsigned long long *d_stats;
+  ENSURE(c->b_scores, (size_t)chunk * E, &d_scores);
+  ENSURE(c->b_res, (size_t)chunk * cap, &d_res);
+  ENSURE(c->b_nres, chunk, &d_nres);
+  ENSURE(c->b_kept, (size_t)chunk * K, &d_kept);
+  ENSURE(c->b_nkeep, chunk, &d_nkeep);
+  ENSURE(c->b_qlist, chunk, &d_qlist);
+  ENSURE(c->b_counter, 64, &d_counter);
+  ENSURE(c->b_stats, 4, &d_stats);
+  HIPCHECK(hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), c->stream));
+  HIPCHECK(hipMemcpyAsync(d_status, c->qstat.data(), (size_t)Q * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemsetAsync(d_nout, 0, (size_t)std::max(Q, 1) * 4, c->stream));
+  double t_s1 = 0, t_pitch = 0, t_ref = 0;
+  long long launches = 0;
+  hipEvent_t ev0, ev1, ev1b, ev2, ev3;
+  HIPCHECK(hipEventCreate(&ev0));
+  HIPCHECK(hipEventCreate(&ev1));
+  HIPCHECK(hipEventCreate(&ev1b));
+  HIPCHECK(hipEventCreate(&ev2));
+  HIPCHECK(hipEventCreate(&ev3));
+  const bool fp32 = precision == PP_FP32;
+  for (int q0 = 0; q0 < Q; q0 += chunk) {
+    const int nq = std::min(chunk, Q - q0);
+    st = fp32 ? launch_stage1<float>(c, q0, nq, prefs->start_pend, d_scores, ev0)
+              : launch_stage1<double>(c, q0, nq, prefs->start_pend, d_scores, ev0);
+    if (st) return st;
+    launches += 2;
+    HIPCHECK(hipEventRecord(ev1, c->stream));
+    // bucket queries of the chunk by masked length
+    std::vector<std::vector<int>> buckets(16);
+    for (int l = 0; l < nq; l++)
+      if (c->qstat[q0 + l] == PP_Q_OK) buckets[bucket_of(c->qM[q0 + l])].push_back(l);
+    std::vector<int> flat;
+    std::vector<int> boff(17, 0);
+    for (int b = 0; b < 16; b++) {
+      boff[b] = (int)flat.size();
+      flat.insert(flat.end(), buckets[b].begin(), buckets[b].end());
+    }
+    boff[16] = (int)flat.size();
+    if (!flat.empty())
+      HIPCHECK(hipMemcpyAsync(d_qlist, flat.data(), flat.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemsetAsync(d_counter, 0, 64 * 4, c->stream));
+    HIPCHECK(hipEventRecord(ev1b, c->stream));
+    const uint8_t *codes = reinterpret_cast<const uint8_t *>(c->b_codes.p) + (size_t)q0 * N;
+    int *status = d_status + q0;
+    for (int b = 0; b < 16; b++) {
+      const int n = boff[b + 1] - boff[b];
+      if (!n) continue;
+      PitchArgs p;
+      p.model = c->d_model;
+      p.codes = codes;
+      p.scores = d_scores;
+      p.qlist = d_qlist + boff[b];
+      p.nq = n;
+      p.counter = d_counter + b;
+      p.darr = fp32 ? c->b_d32.p : c->b_d64.p;
+      p.parr = fp32 ? c->b_p32.p : c->b_p64.p;
+      p.darr_e = reinterpret_cast<const int *>(c->b_de.p);
+      p.parr_e = reinterpret_cast<const int *>(c->b_pe.p);
+      p.bl = reinterpret_cast<const double *>(c->b_bl.p);
+      p.N = N;
+      p.E = E;
+      p.start_pend = prefs->start_pend;
+      p.max_pend = prefs->max_pend;
+      p.initial_tol = prefs->initial_tolerance;
+      p.strike_box = prefs->strike_box;
+      p.t_max_pitches = t_max_pitches;
+      p.t_max_strikes = t_max_strikes;
+      p.cap = cap;
+      p.res = d_res;
+      p.n_res = d_nres;
+      p.status = status;
+      p.n_calls = d_stats;
+      p.n_pitched = d_stats + 1;
+      st = fp32 ? dispatch_pitch<float>(c, p, 64 * (b + 1)) : dispatch_pitch<double>(c, p, 64 * (b + 1));
+      if (st) return st;
+      launches++;
+    }
+    HIPCHECK(hipEventRecord(ev2, c->stream));
+    KeepArgs ka;
+    ka.Q = nq;
+    ka.cap = cap;
+    ka.K = K;
+    ka.res = d_res;
+    ka.n_res = d_nres;
+    ka.status = status;
+    ka.log_keep_factor = std::log(prefs->keep_factor);
+    ka.kept = d_kept;
+    ka.n_keep = d_nkeep;
+    hipLaunchKernelGGL(keep_kernel, dim3((nq + 255) / 256), dim3(256), 0, c->stream, ka);
+    HIPCHECK(hipGetLastError());
+    for (int b = 0; b < 16; b++) {
+      const int n = boff[b + 1] - boff[b];
+      if (!n) continue;
+      OptArgs o;
+      std::memset(&o, 0, sizeof(o));
+      o.model = c->d_model;
+      o.codes = codes;
+      o.N = N;
+      o.mode = 1;
+      o.n_jobs = n * K;
+      o.K = K;
+      o.qlist = d_qlist + boff[b];
+      o.counter = d_counter + 16 + b;
+      o.darr = c->b_d64.p;
+      o.parr = c->b_p64.p;
+      o.darr_e = reinterpret_cast<const int *>(c->b_de.p);
+      o.parr_e = reinterpret_cast<const int *>(c->b_pe.p);
+      o.bl = reinterpret_cast<const double *>(c->b_bl.p);
+      o.tol = 1e-5;  // core.ml:14 final_tolerance
+      o.max_pend = prefs->max_pend;
+      o.start_pend = prefs->start_pend;
+      o.kept = d_kept;
+      o.n_keep = d_nkeep;
+      o.status = status;
+      o.n_calls = d_stats;
+      st = dispatch_opt<double>(c, o, 64 * (b + 1));
+      if (st) return st;
+      launches++;
+    }
+    FinalArgs fa;
+    fa.Q = nq;
+    fa.K = K;
+    fa.kept = d_kept;
+    fa.n_keep = d_nkeep;
+    fa.status = status;
+    fa.out = d_out + (size_t)q0 * K;
+    fa.n_out = d_nout + q0;
+    hipLaunchKernelGGL(finalize_kernel, dim3((nq + 255) / 256), dim3(256), 0, c->stream, fa);
+    HIPCHECK(hipGetLastError());
+    launches += 2;
+    HIPCHECK(hipEventRecord(ev3, c->stream));
+    HIPCHECK(hipEventSynchronize(ev3));
+    float a1 = 0, a2 = 0, a3 = 0;
+    HIPCHECK(hipEventElapsedTime(&a1, ev0, ev1));
+    HIPCHECK(hipEventElapsedTime(&a2, ev1b, ev2));
+    HIPCHECK(hipEventElapsedTime(&a3, ev2, ev3));
+    t_s1 += a1;
+    t_pitch += a2;
+    t_ref += a3;
+  }
+  for (hipEvent_t e : {ev0, ev1, ev1b, ev2, ev3}) (void)hipEventDestroy(e);
+  if (stats) {
+    unsigned long long hs[4];
+    HIPCHECK(hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost));
+    stats->t_stage1_ms = t_s1;
+    stats->t_pitch_ms = t_pitch;
+    stats->t_refine_ms = t_ref;
+    stats->t_ranking_ms = t_s1;
+    stats->t_ml_ms = t_pitch + t_ref;
+    stats->n_logdots = 0;
+    stats->n_site_pairs = 0;
+    for (int q = 0; q < Q; q++)
+      if (c->qstat[q] == PP_Q_OK) {
+        stats->n_logdots += E;
+        stats->n_site_pairs += (int64_t)c->qM[q] * E;
+      }
+    stats->n_like_calls = (int64_t)hs[0];
+    stats->n_pitched = (int64_t)hs[1];
+    stats->n_site_probes_pend = (int64_t)hs[2];
+    stats->n_site_probes_dist = (int64_t)hs[3];
+    stats->n_launches = launches;
+  }
+  return PP_OK;
+}
+
+int results_download(pp_ctx *c, pp_placement *out, int *n_out, int *status) {
+  const int Q = c->Q, K = c->K_last;
+  std::vector<pp_placement> h((size_t)Q * K);
+  std::vector<int> hn(Q), hs(Q);
+  if (Q) {
+    HIPCHECK(hipMemcpy(h.data(), c->b_out.p, h.size() * sizeof(pp_placement), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(hn.data(), c->b_nout.p, Q * 4, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(hs.data(), c->b_status.p, Q * 4, hipMemcpyDeviceToHost));
+  }
+  for (int i = 0; i < Q; i++) {
+    const int q = c->order[i];
+    n_out[q] = hn[i];
+    status[q] = hs[i];
+    std::memcpy(out + (size_t)q * K, h.data() + (size_t)i * K, sizeof(pp_placement) * K);
+  }
+  return PP_OK;
+}
+
+// host <-> device helper for the per-call twins
+struct Scratch {
+  std::vector<void *> ptrs;
+  ~Scratch() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+  template <class T>
+  T *up(const T *h, size_t n) {
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(n * sizeof(T), 8)) != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    if (h && n) (void)hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice);
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int pp_ctx_create(int device, pp_ctx **out) {
+  CHECK(out, PP_EINVAL, "out is null");
+  int n = 0;
+  HIPCHECK(hipGetDeviceCount(&n));
+  CHECK(device >= 0 && device < n, PP_EINVAL, "no such GPU device");
+  pp_ctx *c = new pp_ctx();
+  c->device = device;
+  HIPCHECK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIPCHECK(hipGetDeviceProperties(&prop, device));
+  c->n_cu = prop.multiProcessorCount;
+  HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  *out = c;
+  return PP_OK;
+}
+
+int pp_ctx_destroy(pp_ctx *c) {
+  if (!c) return PP_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  DBuf *bufs[] = {&c->b_model, &c->b_bl, &c->b_d64, &c->b_p64, &c->b_d32, &c->b_p32, &c->b_de,
+                  &c->b_pe, &c->b_s64, &c->b_s32, &c->b_se, &c->b_codes, &c->b_qM, &c->b_tcodes,
+                  &c->b_tile_lo, &c->b_tile_hi, &c->b_tile_q, &c->b_ttab, &c->b_scores, &c->b_res,
+                  &c->b_nres, &c->b_status, &c->b_kept, &c->b_nkeep, &c->b_out, &c->b_nout,
+                  &c->b_qlist, &c->b_counter, &c->b_stats};
+  for (DBuf *b : bufs)
+    if (b->p) (void)hipFree(b->p);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PP_OK;
+}
+
+int pp_model_build(pp_ctx *c, int n_states, int n_rates, const double *exch, const double *statd,
+                   const double *rates) {
+  CHECK(c && exch && statd && rates, PP_EINVAL, "null argument");
+  int st = set_device(c);
+  if (st) return st;
+  std::string err;
+  st = build_model(c->model, n_states, n_rates, exch, statd, rates, err);
+  if (st) { set_error(err); return st; }
+  DevModel dm;
+  c->model.fill_dev(&dm);
+  DevModel *d;
+  ENSURE(c->b_model, 1, &d);
+  HIPCHECK(hipMemcpy(d, &dm, sizeof(dm), hipMemcpyHostToDevice));
+  c->d_model = d;
+  c->has_tree = false;
+  return PP_OK;
+}
+
+int pp_model_get(pp_ctx *c, double *x, double *l, double *xit) {
+  CHECK(c && c->model.built, PP_ESTATE, "no model");
+  const int S = c->model.S;
+  std::memcpy(x, c->model.x.data(), S * S * 8);
+  std::memcpy(l, c->model.l.data(), S * 8);
+  std::memcpy(xit, c->model.xit.data(), S * S * 8);
+  return PP_OK;
+}
+
+int pp_tree_set(pp_ctx *c, int n_nodes, const int *child_start, const int *children,
+                const double *bl, int n_sites, const char *leaf_seqs) {
+  CHECK(c && child_start && children && bl && leaf_seqs, PP_EINVAL, "null argument");
+  CHECK(c->model.built, PP_ESTATE, "no model: call pp_model_build first");
+  CHECK(n_nodes >= 2 && n_sites >= 1, PP_EINVAL, "need n_nodes >= 2 and n_sites >= 1");
+  int st = set_device(c);
+  if (st) return st;
+  TreeDesc t{n_nodes, n_sites, child_start, children, bl};
+  TreeGlvs g;
+  std::string err;
+  st = host_precompute(c->model, t, leaf_seqs, g, err);
+  if (st) { set_error(err); return st; }
+  c->G = n_nodes;
+  c->N = n_sites;
+  c->E = n_nodes - 1;
+  c->Epad = (c->E + 255) / 256 * 256;
+  c->bl.assign(bl, bl + n_nodes);
+  return upload_glvs(c, g);
+}
+
+int pp_tree_upload(pp_ctx *c, int n_nodes, const double *bl, int n_sites, const double *darr,
+                   const int64_t *darr_e, const double *parr, const int64_t *parr_e,
+                   const double *snodes, const int64_t *snodes_e) {
+  CHECK(c && bl && darr && darr_e && parr && parr_e && snodes && snodes_e, PP_EINVAL, "null argument");
+  CHECK(c->model.built, PP_ESTATE, "no model: call pp_model_build first");
+  CHECK(n_nodes >= 2 && n_sites >= 1, PP_EINVAL, "need n_nodes >= 2 and n_sites >= 1");
+  int st = set_device(c);
+  if (st) return st;
+  const size_t L = (size_t)n_nodes * c->model.R * n_sites * c->model.S;
+  TreeGlvs g;
+  g.darr.assign(darr, darr + L);
+  g.parr.assign(parr, parr + L);
+  g.snodes.assign(snodes, snodes + L);
+  g.darr_e.assign(darr_e, darr_e + (size_t)n_nodes * n_sites);
+  g.parr_e.assign(parr_e, parr_e + (size_t)n_nodes * n_sites);
+  g.snodes_e.assign(snodes_e, snodes_e + (size_t)n_nodes * n_sites);
+  c->G = n_nodes;
+  c->N = n_sites;
+  c->E = n_nodes - 1;
+  c->Epad = (c->E + 255) / 256 * 256;
+  c->bl.assign(bl, bl + n_nodes);
+  return upload_glvs(c, g);
+}
+
+int pp_tree_download(pp_ctx *c, double *darr, int64_t *darr_e, double *parr, int64_t *parr_e,
+                     double *snodes, int64_t *snodes_e) {
+  CHECK(c && c->has_tree, PP_ESTATE, "no tree");
+  int st = set_device(c);
+  if (st) return st;
+  const int G = c->G, N = c->N, R = c->model.R, S = c->model.S, RS = R * S, Epad = c->Epad;
+  std::vector<double> a((size_t)G * N * RS), b((size_t)G * N * RS), s((size_t)N * Epad * RS);
+  std::vector<int> ea((size_t)G * N), eb((size_t)G * N), es((size_t)N * Epad);
+  HIPCHECK(hipMemcpy(a.data(), c->b_d64.p, a.size() * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(b.data(), c->b_p64.p, b.size() * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(s.data(), c->b_s64.p, s.size() * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(ea.data(), c->b_de.p, ea.size() * 4, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(eb.data(), c->b_pe.p, eb.size() * 4, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(es.data(), c->b_se.p, es.size() * 4, hipMemcpyDeviceToHost));
+  for (int i = 0; i < G; i++)
+    for (int r = 0; r < R; r++)
+      for (int n = 0; n < N; n++)
+        for (int k = 0; k < S; k++) {
+          const size_t dst = (((size_t)i * R + r) * N + n) * S + k;
+          const size_t src = ((size_t)i * N + n) * RS + r * S + k;
+          if (darr) darr[dst] = a[src];
+          if (parr) parr[dst] = b[src];
+          if (snodes) snodes[dst] = i < G - 1 ? s[((size_t)n * Epad + i) * RS + r * S + k] : 0.0;
+        }
+  for (int i = 0; i < G; i++)
+    for (int n = 0; n < N; n++) {
+      if (darr_e) darr_e[(size_t)i * N + n] = ea[(size_t)i * N + n];
+      if (parr_e) parr_e[(size_t)i * N + n] = eb[(size_t)i * N + n];
+      if (snodes_e) snodes_e[(size_t)i * N + n] = i < G - 1 ? es[(size_t)n * Epad + i] : 0;
+    }
+  return PP_OK;
+}
+
+int pp_queries_upload(pp_ctx *c, int n_queries, const char *queries) {
+  CHECK(c && (queries || n_queries == 0), PP_EINVAL, "null argument");
+  int st = set_device(c);
+  if (st) return st;
+  return queries_upload(c, n_queries, queries);
+}
+
+int pp_place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats) {
+  CHECK(c, PP_EINVAL, "null context");
+  CHECK(c->has_tree, PP_ESTATE, "no tree");
+  return place_resident(c, prefs, precision, stats);
+}
+
+int pp_results_download(pp_ctx *c, pp_placement *out, int *n_out, int *status) {
+  CHECK(c && out && n_out && status, PP_EINVAL, "null argument");
+  int st = set_device(c);
+  if (st) return st;
+  return results_download(c, out, n_out, status);
+}
+
+int pp_synchronize(pp_ctx *c) {
+  CHECK(c, PP_EINVAL, "null context");
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return PP_OK;
+}
+
+int pp_place_batch(pp_ctx *c, int n_queries, const char *queries, const pp_prefs *prefs,
+                   int precision, pp_placement *out, int *n_out, int *status, pp_stats *stats) {
+  int st = pp_queries_upload(c, n_queries, queries);
+  if (st) return st;
+  st = pp_place_resident(c, prefs, precision, stats);
+  if (st) return st;
+  st = pp_results_download(c, out, n_out, status);
+  if (st) return st;
+  for (int q = 0; q < n_queries; q++)
+    if (status[q] != PP_Q_OK) {
+      set_error("query " + std::to_string(q) + " failed with status " + std::to_string(status[q]));
+      return PP_EQUERY;
+    }
+  return PP_OK;
+}
+
+int pp_stage1_scores(pp_ctx *c, int n_queries, const char *queries, double start_pend,
+                     int precision, double *scores) {
+  CHECK(c && scores, PP_EINVAL, "null argument");
+  CHECK(precision == PP_FP64 || precision == PP_FP32, PP_EINVAL, "bad precision");
+  int st = pp_queries_upload(c, n_queries, queries);
+  if (st) return st;
+  const int E = c->E;
+  double *d;
+  ENSURE(c->b_scores, (size_t)std::max(n_queries, 1) * E, &d);
+  st = precision == PP_FP32 ? launch_stage1<float>(c, 0, n_queries, start_pend, d)
+                            : launch_stage1<double>(c, 0, n_queries, start_pend, d);
+  if (st) return st;
+  std::vector<double> h((size_t)n_queries * E);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipMemcpy(h.data(), d, h.size() * 8, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n_queries; i++) {
+    const int q = c->order[i];
+    if (c->qstat[i] != PP_Q_OK) {
+      for (int e = 0; e < E; e++) scores[(size_t)q * E + e] = NAN;
+    } else {
+      std::memcpy(scores + (size_t)q * E, h.data() + (size_t)i * E, (size_t)E * 8);
+    }
+  }
+  return PP_OK;
+}
+
+int pp_rank_topk(pp_ctx *c, int n_queries, int n_edges, const double *scores, int k, int *ranked) {
+  CHECK(c && scores && ranked, PP_EINVAL, "null argument");
+  CHECK(n_edges >= 1 && k >= 1 && k <= n_edges, PP_EINVAL, "need 1 <= k <= n_edges");
+  int st = set_device(c);
+  if (st) return st;
+  Scratch s;
+  double *d_s = s.up(scores, (size_t)n_queries * n_edges);
+  int *d_r = s.up<int>(nullptr, (size_t)n_queries * k);
+  CHECK(d_s && d_r, PP_ENOMEM, "device allocation failed");
+  const size_t lds = 4 * ((n_edges + 31) / 32);
+  hipLaunchKernelGGL(topk_kernel, dim3(std::max(n_queries, 1)), dim3(256), lds, c->stream, d_s, n_queries,
+                     n_edges, k, d_r);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipMemcpy(ranked, d_r, (size_t)n_queries * k * 4, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+int pp_optimize_batch(pp_ctx *c, int n_jobs, const char *queries, const int *query_of_job,
+                      const int *locs, const double *pend0, const double *dist0, double tolerance,
+                      double max_pend, int precision, double *ll, double *pend, double *dist,
+                      int *n_calls, int *status) {
+  CHECK(c && queries && query_of_job && locs && pend0 && dist0 && ll && pend && dist && n_calls && status,
+        PP_EINVAL, "null argument");
+  CHECK(precision == PP_FP64 || precision == PP_FP32, PP_EINVAL, "bad precision");
+  int n_queries = 0;
+  for (int j = 0; j < n_jobs; j++) n_queries = std::max(n_queries, query_of_job[j] + 1);
+  int st = pp_queries_upload(c, n_queries, queries);
+  if (st) return st;
+  for (int j = 0; j < n_jobs; j++) CHECK(locs[j] >= 0 && locs[j] < c->E, PP_EINVAL, "loc out of range");
+  std::vector<int> pos(n_queries);
+  for (int i = 0; i < n_queries; i++) pos[c->order[i]] = i;
+  std::vector<int> jq(n_jobs);
+  for (int j = 0; j < n_jobs; j++) jq[j] = pos[query_of_job[j]];
+  Scratch s;
+  int *d_jq = s.up(jq.data(), n_jobs), *d_loc = s.up(locs, n_jobs);
+  double *d_p0 = s.up(pend0, n_jobs), *d_d0 = s.up(dist0, n_jobs);
+  double *d_ll = s.up<double>(nullptr, n_jobs), *d_pe = s.up<double>(nullptr, n_jobs),
+         *d_di = s.up<double>(nullptr, n_jobs);
+  int *d_calls = s.up<int>(nullptr, n_jobs), *d_st = s.up<int>(nullptr, n_jobs);
+  unsigned long long *d_stats = s.up<unsigned long long>(nullptr, 4);
+  int *d_counter = s.up<int>(nullptr, 16);
+  CHECK(d_stats && d_counter, PP_ENOMEM, "device allocation failed");
+  std::vector<int> init_st(n_jobs, PP_Q_OK);
+  for (int j = 0; j < n_jobs; j++)
+    if (c->qstat[jq[j]] != PP_Q_OK) init_st[j] = 100 + c->qstat[jq[j]];
+  HIPCHECK(hipMemcpy(d_st, init_st.data(), n_jobs * 4, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemset(d_counter, 0, 64));
+  HIPCHECK(hipMemset(d_stats, 0, 32));
+  std::vector<std::vector<int>> buckets(16);
+  for (int j = 0; j < n_jobs; j++)
+    if (init_st[j] == PP_Q_OK) buckets[bucket_of(c->qM[jq[j]])].push_back(j);
+  std::vector<int> flat;
+  std::vector<int> boff(17, 0);
+  for (int b = 0; b < 16; b++) {
+    boff[b] = (int)flat.size();
+    flat.insert(flat.end(), buckets[b].begin(), buckets[b].end());
+  }
+  boff[16] = (int)flat.size();
+  int *d_list = s.up(flat.data(), flat.size());
+  for (int b = 0; b < 16; b++) {
+    const int n = boff[b + 1] - boff[b];
+    if (!n) continue;
+    OptArgs o;
+    std::memset(&o, 0, sizeof(o));
+    o.model = c->d_model;
+    o.codes = reinterpret_cast<const uint8_t *>(c->b_codes.p);
+    o.N = c->N;
+    o.mode = 0;
+    o.n_jobs = n;
+    o.K = 1;
+    o.qlist = d_list + boff[b];
+    o.counter = d_counter + b;
+    const bool fp32 = precision == PP_FP32;
+    o.darr = fp32 ? c->b_d32.p : c->b_d64.p;
+    o.parr = fp32 ? c->b_p32.p : c->b_p64.p;
+    o.darr_e = reinterpret_cast<const int *>(c->b_de.p);
+    o.parr_e = reinterpret_cast<const int *>(c->b_pe.p);
+    o.bl = reinterpret_cast<const double *>(c->b_bl.p);
+    o.tol = tolerance;
+    o.max_pend = max_pend;
+    o.start_pend = 0.1;
+    o.job_q = d_jq;
+    o.job_loc = d_loc;
+    o.job_pend = d_p0;
+    o.job_dist = d_d0;
+    o.out_ll = d_ll;
+    o.out_pend = d_pe;
+    o.out_dist = d_di;
+    o.out_calls = d_calls;
+    o.out_status = d_st;
+    o.n_calls = d_stats;
+    st = fp32 ? dispatch_opt<float>(c, o, 64 * (b + 1)) : dispatch_opt<double>(c, o, 64 * (b + 1));
+    if (st) return st;
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipMemcpy(ll, d_ll, n_jobs * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(pend, d_pe, n_jobs * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(dist, d_di, n_jobs * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(n_calls, d_calls, n_jobs * 4, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(status, d_st, n_jobs * 4, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+// ---- linear_c.c twins ------------------------------------------------------
+
+int pp_gemmish(double *dst, const double *a, const double *b, int n_sites, int n_states) {
+  CHECK(dst && a && b && n_sites >= 0 && n_states > 0, PP_EINVAL, "bad arguments");
+  Scratch s;
+  const size_t n = (size_t)n_sites * n_states;
+  double *da = s.up(a, (size_t)n_states * n_states), *db = s.up(b, n), *dd = s.up<double>(nullptr, n);
+  CHECK(da && db && dd, PP_ENOMEM, "device allocation failed");
+  if (n) hipLaunchKernelGGL(gemmish_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, dd, da, db, n_sites, n_states);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpy(dst, dd, n * 8, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+int pp_dediagonalize(double *dst, const double *u, const double *lambda, const double *uit, int n) {
+  CHECK(dst && u && lambda && uit && n > 0, PP_EINVAL, "bad arguments");
+  Scratch s;
+  double *du = s.up(u, (size_t)n * n), *dl = s.up(lambda, n), *dv = s.up(uit, (size_t)n * n),
+         *dd = s.up<double>(nullptr, (size_t)n * n);
+  CHECK(du && dl && dv && dd, PP_ENOMEM, "device allocation failed");
+  hipLaunchKernelGGL(dediag_kernel, dim3((n * n + 255) / 256), dim3(256), 0, 0, dd, du, dl, dv, n);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpy(dst, dd, (size_t)n * n * 8, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+static int twin_prod(double *dst, const double *statd, const double *x, const double *y, int R, int N,
+                     int S) {
+  CHECK(dst && x && y && R > 0 && N >= 0 && S > 0, PP_EINVAL, "bad arguments");
+  Scratch s;
+  const size_t n = (size_t)R * N * S;
+  double *dx = s.up(x, n), *dy = s.up(y, n), *dd = s.up<double>(nullptr, n);
+  double *dst_d = statd ? s.up(statd, S) : nullptr;
+  CHECK(dx && dy && dd, PP_ENOMEM, "device allocation failed");
+  if (n) hipLaunchKernelGGL(pairwise_kernel, dim3(std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0, 0, dd,
+                            dx, dy, (long)n, dst_d, S);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpy(dst, dd, n * 8, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+int pp_ten_pairwise_prod(double *dst, const double *x, const double *y, int n_rates, int n_sites,
+                         int n_states) {
+  return twin_prod(dst, nullptr, x, y, n_rates, n_sites, n_states);
+}
+
+int pp_ten_statd_pairwise_prod(const double *statd, double *dst, const double *a, const double *b,
+                               int n_rates, int n_sites, int n_states) {
+  CHECK(statd, PP_EINVAL, "statd is null");
+  return twin_prod(dst, statd, a, b, n_rates, n_sites, n_states);
+}
+
+static int twin_logdot(const double *statd, const double *x, const double *y, const double *z,
+                       const uint16_t *mask, int first, int last, double *util, int R, int N, int S,
+                       double *out) {
+  CHECK(x && y && out && R > 0 && N >= 0 && S > 0, PP_EINVAL, "bad arguments");
+  CHECK(first >= 0 && last < N && first <= last + 1, PP_EINVAL, "bad site interval");
+  const int n_used = last - first + 1;
+  Scratch s;
+  const size_t n = (size_t)R * N * S;
+  double *dx = s.up(x, n), *dy = s.up(y, n), *dz = z ? s.up(z, n) : nullptr;
+  double *ds = statd ? s.up(statd, S) : nullptr;
+  uint16_t *dm = mask ? s.up(mask, N) : nullptr;
+  double *du = s.up<double>(nullptr, std::max(n_used, 1)), *dout = s.up<double>(nullptr, 1);
+  CHECK(dx && dy && du && dout, PP_ENOMEM, "device allocation failed");
+  if (n_used > 0)
+    hipLaunchKernelGGL(util_kernel, dim3((n_used + 255) / 256), dim3(256), 0, 0, du, ds, dx, dy, dz, dm, first,
+                       n_used, R, N, S);
+  hipLaunchKernelGGL(logsum_kernel, dim3(1), dim3(64), 0, 0, du, dm, first, n_used, R, dout);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpy(out, dout, 8, hipMemcpyDeviceToHost));
+  if (util && n_used > 0) HIPCHECK(hipMemcpy(util, du, (size_t)n_used * 8, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+int pp_ten_log_like3(const double *statd, const double *x, const double *y, const double *z,
+                     double *util, int n_rates, int n_sites, int n_states, double *out) {
+  CHECK(statd && z, PP_EINVAL, "null argument");
+  return twin_logdot(statd, x, y, z, nullptr, 0, n_sites - 1, util, n_rates, n_sites, n_states, out);
+}
+
+int pp_ten_masked_logdot(const double *x, const double *y, const uint16_t *mask, double *util,
+                         int n_rates, int n_sites, int n_states, double *out) {
+  CHECK(mask, PP_EINVAL, "mask is null");
+  return twin_logdot(nullptr, x, y, nullptr, mask, 0, n_sites - 1, util, n_rates, n_sites, n_states, out);
+}
+
+int pp_ten_bounded_logdot(const double *x, const double *y, int first, int last, double *util,
+                          int n_rates, int n_sites, int n_states, double *out) {
+  return twin_logdot(nullptr, x, y, nullptr, nullptr, first, last, util, n_rates, n_sites, n_states, out);
+}
+
+}  // extern "C"

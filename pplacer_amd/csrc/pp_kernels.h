@@ -1,0 +1,90 @@
+// Kernel argument blocks and kernel declarations (pp_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pp_internal.h"
+
+namespace pp {
+
+struct Res {  // one optimised (query, edge): Three_tax.get_results + location
+  double ll, pend, dist;
+  int loc;
+  int flags;
+};
+
+struct Stage1Args {
+  const void *sn;         // [N][Epad][R*4] Real
+  int N, E, Epad, n_codes, n_inf;
+  const int *sn_e;        // [N][Epad]
+  const uint8_t *tcodes;  // [n_tiles][N][TQ]
+  const int *tile_lo, *tile_hi, *tile_q;
+  const int *qM;          // masked length per (batch-local) query
+  const double *ttab;     // [n_codes][R*4] query rows evolved by start_pend
+  double log_R;
+  double *scores;         // [Q][E]
+};
+
+struct PitchArgs {
+  const DevModel *model;
+  const uint8_t *codes;  // [Q][N]
+  const double *scores;  // [Q][E]
+  const int *qlist;
+  int nq;
+  int *counter;
+  const void *darr, *parr;
+  const int *darr_e, *parr_e;
+  const double *bl;
+  int N, E;
+  double start_pend, max_pend, initial_tol, strike_box;
+  long long t_max_pitches, t_max_strikes;
+  int cap;
+  Res *res;  // [Q][cap]
+  int *n_res, *status;
+  unsigned long long *n_calls, *n_pitched;
+};
+
+struct KeepArgs {
+  int Q, cap, K;
+  const Res *res;
+  const int *n_res;
+  const int *status;
+  double log_keep_factor;
+  Res *kept;  // [Q][K]
+  int *n_keep;
+};
+
+struct OptArgs {
+  const DevModel *model;
+  const uint8_t *codes;
+  int N, mode, n_jobs, K;
+  const int *qlist;  // mode 1: bucket queries; mode 0: optional job indirection
+  int *counter;
+  const void *darr, *parr;
+  const int *darr_e, *parr_e;
+  const double *bl;
+  double tol, max_pend, start_pend;
+  // mode 1
+  Res *kept;
+  const int *n_keep;
+  int *status;
+  // mode 0
+  const int *job_q, *job_loc;
+  const double *job_pend, *job_dist;
+  double *out_ll, *out_pend, *out_dist;
+  int *out_calls, *out_status;
+  unsigned long long *n_calls;
+};
+
+struct FinalArgs {
+  int Q, K;
+  const Res *kept;
+  const int *n_keep;
+  const int *status;
+  pp_placement *out;
+  int *n_out;
+};
+
+}  // namespace pp

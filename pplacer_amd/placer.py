@@ -1,0 +1,247 @@
+"""Host-side mirror of pplacer's placement driver over libpplacer_hip.so.
+
+`Placer` follows Pplacer_run.run_placements (pplacer_src/pplacer_run.ml:167-656)
+and Core.pplacer_core (pplacer_src/core.ml:76-411) for the gamma mixture model:
+build the model from the reference alignment, pre-mask, compute the
+reference-tree GLVs, then place batches of queries.  Every numeric step runs
+in the C++/HIP library; this module only moves arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import refio
+from ._lib import (PP_FP32, PP_FP64, Placement, Prefs, Stats, check, lib)
+
+_dp = C.POINTER(C.c_double)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def default_prefs(**kw) -> Prefs:
+    """prefs.ml:65-121 defaults (start_pend 0.1, max_pend 2, initial_tolerance
+    0.01, strike_box 3, keep_factor 0.01, max_strikes 6, max_pitches 40,
+    keep_at_most 7)."""
+    p = Prefs()
+    lib().pp_prefs_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def discrete_gamma(n_cat: int, alpha: float) -> np.ndarray:
+    out = np.empty(n_cat)
+    check(lib().pp_discrete_gamma(n_cat, alpha, _p(out)), "discrete_gamma")
+    return out
+
+
+def emper_freq(n_states: int, seqs) -> np.ndarray:
+    n_sites = len(seqs[0])
+    out = np.empty(n_states)
+    check(lib().pp_emper_freq(n_states, "".join(seqs).encode(), len(seqs), n_sites, _p(out)),
+          "emper_freq")
+    return out
+
+
+def premask(n_states: int, ref_seqs, query_seqs, discard_nonoverlapped=False):
+    """(mask, query_keep, masklen) as Pplacer_run.premask computes them."""
+    n_sites = len(ref_seqs[0])
+    mask = np.zeros(n_sites, dtype=np.uint8)
+    keep = np.zeros(max(len(query_seqs), 1), dtype=np.uint8)
+    ml = C.c_int(0)
+    check(lib().pp_premask(n_states, "".join(ref_seqs).encode(), len(ref_seqs),
+                           "".join(query_seqs).encode(), len(query_seqs), n_sites,
+                           int(discard_nonoverlapped), _p(mask, C.c_uint8), _p(keep, C.c_uint8),
+                           C.byref(ml)), "premask")
+    return mask.astype(bool), keep[: len(query_seqs)].astype(bool), ml.value
+
+
+def cut_columns(seqs, mask):
+    m = np.asarray(mask, dtype=bool)
+    out = []
+    for s in seqs:
+        a = np.frombuffer(s.encode(), dtype=np.uint8)
+        out.append(a[m].tobytes().decode())
+    return out
+
+
+@dataclass
+class PlacementRecord:
+    """Placement.make_ml fields (placement.ml:50-60)."""
+
+    edge_num: int
+    like_weight_ratio: float
+    likelihood: float
+    distal_length: float
+    pendant_length: float
+
+
+class Placer:
+    """One GPU context holding a model, a reference tree and query batches."""
+
+    def __init__(self, device: int = 0):
+        self._ctx = C.c_void_p()
+        check(lib().pp_ctx_create(device, C.byref(self._ctx)), "pp_ctx_create")
+        self.n_states = None
+        self.n_rates = None
+        self.tree = None
+        self.n_sites = None
+
+    def close(self):
+        if self._ctx:
+            lib().pp_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- model (gmix_model.ml:25-37) -------------------------------------
+    def build_model(self, exch, statd, rates):
+        exch, statd, rates = _f64(exch), _f64(statd), _f64(rates)
+        S, R = statd.shape[0], rates.shape[0]
+        check(lib().pp_model_build(self._ctx, S, R, _p(exch), _p(statd), _p(rates)), "model_build")
+        self.n_states, self.n_rates = S, R
+
+    def model_diagd(self):
+        S = self.n_states
+        x, l, xit = np.empty((S, S)), np.empty(S), np.empty((S, S))
+        check(lib().pp_model_get(self._ctx, _p(x), _p(l), _p(xit)), "model_get")
+        return x, l, xit
+
+    # ---- tree ------------------------------------------------------------
+    def set_tree(self, tree: refio.Tree, leaf_seqs: dict, n_sites: int):
+        """leaf_seqs: leaf id -> (pre-masked) sequence."""
+        starts, children = tree.csr()
+        G = tree.n_nodes
+        rows = np.full((G, n_sites), ord("-"), dtype=np.uint8)
+        for i, s in leaf_seqs.items():
+            rows[i] = np.frombuffer(s.encode(), dtype=np.uint8)
+        bl = _f64(tree.bl)
+        check(lib().pp_tree_set(self._ctx, G, _p(starts, C.c_int), _p(children, C.c_int), _p(bl),
+                                n_sites, rows.tobytes()), "tree_set")
+        self.tree, self.n_sites = tree, n_sites
+
+    def upload_glvs(self, tree, n_sites, darr, darr_e, parr, parr_e, snodes, snodes_e):
+        G = tree.n_nodes
+        args = [_f64(darr), np.ascontiguousarray(darr_e, np.int64), _f64(parr),
+                np.ascontiguousarray(parr_e, np.int64), _f64(snodes),
+                np.ascontiguousarray(snodes_e, np.int64)]
+        bl = _f64(tree.bl)
+        check(lib().pp_tree_upload(self._ctx, G, _p(bl), n_sites, _p(args[0]), _p(args[1], C.c_int64),
+                                   _p(args[2]), _p(args[3], C.c_int64), _p(args[4]),
+                                   _p(args[5], C.c_int64)), "tree_upload")
+        self.tree, self.n_sites = tree, n_sites
+
+    def download_glvs(self):
+        G, N, R, S = self.tree.n_nodes, self.n_sites, self.n_rates, self.n_states
+        shp, eshp = (G, R, N, S), (G, N)
+        d, p, s = np.empty(shp), np.empty(shp), np.empty(shp)
+        de, pe, se = (np.empty(eshp, dtype=np.int64) for _ in range(3))
+        check(lib().pp_tree_download(self._ctx, _p(d), _p(de, C.c_int64), _p(p), _p(pe, C.c_int64),
+                                     _p(s), _p(se, C.c_int64)), "tree_download")
+        return d, de, p, pe, s, se
+
+    # ---- stage 1 / ranking / stage 2 -------------------------------------
+    def stage1_scores(self, queries, start_pend=0.1, precision=PP_FP64):
+        E = self.tree.n_nodes - 1
+        Q = len(queries)
+        out = np.empty((Q, E))
+        check(lib().pp_stage1_scores(self._ctx, Q, "".join(queries).encode(), start_pend, precision,
+                                     _p(out)), "stage1_scores")
+        return out
+
+    def rank_topk(self, scores, k):
+        scores = _f64(scores)
+        Q, E = scores.shape
+        out = np.empty((Q, k), dtype=np.int32)
+        check(lib().pp_rank_topk(self._ctx, Q, E, _p(scores), k, _p(out, C.c_int)), "rank_topk")
+        return out
+
+    def optimize(self, queries, query_of_job, locs, pend0, dist0, tolerance, max_pend=2.0,
+                 precision=PP_FP64):
+        n = len(locs)
+        qj = np.ascontiguousarray(query_of_job, np.int32)
+        lc = np.ascontiguousarray(locs, np.int32)
+        p0, d0 = _f64(pend0), _f64(dist0)
+        ll, pe, di = np.empty(n), np.empty(n), np.empty(n)
+        calls, st = np.empty(n, np.int32), np.empty(n, np.int32)
+        check(lib().pp_optimize_batch(self._ctx, n, "".join(queries).encode(), _p(qj, C.c_int),
+                                      _p(lc, C.c_int), _p(p0), _p(d0), tolerance, max_pend,
+                                      precision, _p(ll), _p(pe), _p(di), _p(calls, C.c_int),
+                                      _p(st, C.c_int)), "optimize_batch")
+        return ll, pe, di, calls, st
+
+    def place(self, queries, prefs: Prefs | None = None, precision=PP_FP64, strict=False):
+        """Core.process_query for every query.  Returns (placements, status,
+        stats): placements[q] is a list of PlacementRecord sorted by
+        like_weight_ratio descending."""
+        prefs = prefs or default_prefs()
+        Q = len(queries)
+        K = max(1, prefs.keep_at_most)
+        out = (Placement * max(1, Q * K))()
+        n_out = np.zeros(max(Q, 1), np.int32)
+        status = np.zeros(max(Q, 1), np.int32)
+        stats = Stats()
+        self.upload_queries(queries)
+        self.place_resident(prefs, precision, stats)
+        check(lib().pp_results_download(self._ctx, out, _p(n_out, C.c_int), _p(status, C.c_int)),
+              "results_download")
+        res = []
+        for q in range(Q):
+            res.append([PlacementRecord(out[q * K + i].loc, out[q * K + i].ml_ratio,
+                                        out[q * K + i].log_like, out[q * K + i].dist_bl,
+                                        out[q * K + i].pend_bl) for i in range(n_out[q])])
+        if strict and np.any(status[:Q] != 0):
+            bad = int(np.nonzero(status[:Q])[0][0])
+            raise RuntimeError("query %d failed with status %d" % (bad, status[bad]))
+        return res, status[:Q].copy(), stats
+
+    def upload_queries(self, queries):
+        check(lib().pp_queries_upload(self._ctx, len(queries), "".join(queries).encode()),
+              "queries_upload")
+
+    def place_resident(self, prefs: Prefs, precision=PP_FP64, stats: Stats | None = None):
+        stats = stats if stats is not None else Stats()
+        check(lib().pp_place_resident(self._ctx, C.byref(prefs), precision, C.byref(stats)),
+              "place_resident")
+        return stats
+
+    def synchronize(self):
+        check(lib().pp_synchronize(self._ctx), "synchronize")
+
+
+def setup_from_workload(wl, device=0, premasked=True):
+    """Model + pre-mask + tree for a synth.Workload, as run_placements does it
+    (model from the un-masked reference alignment; pplacer_run.ml:200-205)."""
+    S = wl.n_states
+    ref_seqs = [s for _, s in wl.ref_aln]
+    statd = emper_freq(S, ref_seqs)
+    rates = discrete_gamma(wl.n_cats, wl.alpha)
+    queries = [s for _, s in wl.queries]
+    leaf = wl.leaf_seqs()
+    if premasked:
+        mask, keep, n = premask(S, ref_seqs, queries)
+        queries = cut_columns(queries, mask)
+        cut_leaf = cut_columns(list(leaf.values()), mask)
+        leaf = dict(zip(leaf.keys(), cut_leaf))
+    n_sites = len(next(iter(leaf.values())))
+    pl = Placer(device)
+    pl.build_model(wl.exch, statd, rates)
+    pl.set_tree(wl.tree, leaf, n_sites)
+    return pl, queries, statd, rates
+
+
+__all__ = ["Placer", "PlacementRecord", "default_prefs", "discrete_gamma", "emper_freq",
+           "premask", "cut_columns", "setup_from_workload", "PP_FP32", "PP_FP64"]

@@ -156,16 +156,17 @@ def make_workload(name: str, seed: int = 1, n_queries: int | None = None,
     gap = ord("-")
     for k in range(nq):
         v = int(locs[k])
-        mid = _evolve_states(rng, st[v], site_rate * (tree.bl[v] / 2.0), lam, X, Xi)
+        a, b = int(starts[k]), int(starts[k] + lens[k])
+        sr = site_rate[a:b]
+        mid = _evolve_states(rng, st[v, a:b], sr * (tree.bl[v] / 2.0), lam, X, Xi)
         if qmode == "pendant":
-            q = _evolve_states(rng, mid, site_rate * rng.exponential(qpar), lam, X, Xi)
+            q = _evolve_states(rng, mid, sr * rng.exponential(qpar), lam, X, Xi)
         else:
             q = mid.copy()
-            flip = rng.random(ncol) < qpar
+            flip = rng.random(b - a) < qpar
             q[flip] = (q[flip] + rng.integers(1, S, size=int(flip.sum()))) % S
         row = np.full(ncol, gap, dtype=np.uint8)
-        a, b = int(starts[k]), int(starts[k] + lens[k])
-        row[a:b] = abet[q[a:b]]
+        row[a:b] = abet[q]
         queries.append(("q%d" % k, row.tobytes().decode()))
     return Workload(name=name, seed=seed, n_states=S, tree=tree, ref_aln=ref_aln,
                     queries=queries, exch=exch, statd=statd, alpha=alpha, n_cats=4,
