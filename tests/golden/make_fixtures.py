@@ -51,6 +51,18 @@ def prot_tables():
         json.dump(out, fh)
 
 
+def copy_jplace():
+    """Placement files that tests/pplacer/test_placefile.ml:1-24 round-trips."""
+    for d, names in {"simple": ["test1.jplace", "test2.jplace", "test3.jplace"],
+                     "psbA": ["DCM.jplace", "coastal.jplace", "surface.jplace", "upwelling.jplace"]}.items():
+        os.makedirs(os.path.join(HERE, "jplace", d), exist_ok=True)
+        for n in names:
+            dst = os.path.join(HERE, "jplace", d, n)
+            shutil.copy(os.path.join(REF, "tests/data", d, n), dst)
+            os.chmod(dst, 0o644)
+
+
 if __name__ == "__main__":
     copy_like()
     prot_tables()
+    copy_jplace()
