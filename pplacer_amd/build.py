@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpplacer_hip.so")
 
 HIP_SOURCES = ["pp_capi.hip"]
-CPP_SOURCES = ["pp_host.cpp", "pp_tree.cpp"]
+CPP_SOURCES = ["pp_host.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
           "-I", os.path.join(ROOT, "include"), "-I", CSRC]

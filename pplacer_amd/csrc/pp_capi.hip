@@ -193,6 +193,162 @@ int upload_glvs(pp_ctx *c, const TreeGlvs &g) {
   return PP_OK;
 }
 
+// GPU reference-tree precompute (see pre:: kernels).  Node levels come from
+// the post-order ids: heights for the distal pass, depths for the proximal.
+template <int R, int S>
+int gpu_precompute_t(pp_ctx *c, const TreeDesc &t, const std::vector<uint8_t> &codes,
+                     const std::vector<int> &parent, const std::vector<int> &height,
+                     const std::vector<int> &depth) {
+  constexpr int RS = R * S;
+  const int G = c->G, N = c->N, E = c->E, Epad = c->Epad;
+  const size_t glv = (size_t)G * N * RS, snn = (size_t)N * Epad * RS;
+  double *d64, *p64, *s64, *P, *Ph, *lv, *statd;
+  float *d32, *p32, *s32;
+  int *de, *pe, *se, *cs, *ch, *par, *lvl;
+  uint8_t *lc;
+  ENSURE(c->b_d64, glv, &d64);
+  ENSURE(c->b_p64, glv, &p64);
+  ENSURE(c->b_s64, snn, &s64);
+  ENSURE(c->b_d32, glv, &d32);
+  ENSURE(c->b_p32, glv, &p32);
+  ENSURE(c->b_s32, snn, &s32);
+  ENSURE(c->b_de, (size_t)G * N, &de);
+  ENSURE(c->b_pe, (size_t)G * N, &pe);
+  ENSURE(c->b_se, (size_t)N * Epad, &se);
+  DBuf b_evd, b_evde, b_P, b_Ph, b_cs, b_ch, b_par, b_lvl, b_lc, b_lv, b_sd;
+  struct Freer {
+    std::vector<DBuf *> v;
+    ~Freer() { for (DBuf *b : v) if (b->p) (void)hipFree(b->p); }
+  } freer{{&b_evd, &b_evde, &b_P, &b_Ph, &b_cs, &b_ch, &b_par, &b_lvl, &b_lc, &b_lv, &b_sd}};
+  double *evd;
+  int *evde;
+  ENSURE(b_evd, glv, &evd);
+  ENSURE(b_evde, (size_t)G * N, &evde);
+  ENSURE(b_P, (size_t)G * RS * S, &P);
+  ENSURE(b_Ph, (size_t)G * RS * S, &Ph);
+  ENSURE(b_cs, G + 1, &cs);
+  ENSURE(b_ch, std::max(1, t.child_start[G]), &ch);
+  ENSURE(b_par, G, &par);
+  ENSURE(b_lvl, G, &lvl);
+  ENSURE(b_lc, (size_t)G * N, &lc);
+  ENSURE(b_lv, c->model.lv.size(), &lv);
+  ENSURE(b_sd, S, &statd);
+  // transition matrices P(bl) and P(bl/2) for every node, FP64 on the host
+  std::vector<double> hP((size_t)G * RS * S), hPh((size_t)G * RS * S);
+  for (int i = 0; i < G; i++) {
+    multi_exp(c->model, t.bl[i], hP.data() + (size_t)i * RS * S);
+    multi_exp(c->model, t.bl[i] / 2., hPh.data() + (size_t)i * RS * S);
+  }
+  HIPCHECK(hipMemcpy(P, hP.data(), hP.size() * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(Ph, hPh.data(), hPh.size() * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(cs, t.child_start, (G + 1) * 4, hipMemcpyHostToDevice));
+  if (t.child_start[G] > 0) HIPCHECK(hipMemcpy(ch, t.children, t.child_start[G] * 4, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(par, parent.data(), G * 4, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(lc, codes.data(), codes.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(lv, c->model.lv.data(), c->model.lv.size() * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(statd, c->model.statd.data(), S * 8, hipMemcpyHostToDevice));
+  pre::TreeArgs a;
+  a.N = N;
+  a.n_nodes = G;
+  a.child_start = cs;
+  a.children = ch;
+  a.parent = par;
+  a.P = P;
+  a.Phalf = Ph;
+  a.leaf_codes = lc;
+  a.lv = lv;
+  a.darr = d64;
+  a.parr = p64;
+  a.evd = evd;
+  a.darr_e = de;
+  a.parr_e = pe;
+  a.evd_e = evde;
+  // post-order by height
+  const int max_h = *std::max_element(height.begin(), height.end());
+  std::vector<int> order(G);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return height[x] < height[y]; });
+  HIPCHECK(hipMemcpy(lvl, order.data(), G * 4, hipMemcpyHostToDevice));
+  for (int h = 0, pos = 0; h <= max_h; h++) {
+    int n = 0;
+    while (pos + n < G && height[order[pos + n]] == h) n++;
+    if (!n) continue;
+    a.level_nodes = lvl + pos;
+    a.n_level = n;
+    const long work = (long)n * N;
+    hipLaunchKernelGGL((pre::distal_level_kernel<RS, S>), dim3((work + 255) / 256), dim3(256), 0, c->stream, a);
+    HIPCHECK(hipGetLastError());
+    pos += n;
+  }
+  hipLaunchKernelGGL(pre::fill_unit_kernel, dim3(256), dim3(256), 0, c->stream, p64 + (size_t)(G - 1) * N * RS,
+                     pe + (size_t)(G - 1) * N, N, RS);
+  HIPCHECK(hipGetLastError());
+  // pre-order by depth (children of depth-d nodes are at depth d+1)
+  const int max_d = *std::max_element(depth.begin(), depth.end());
+  std::vector<int> dorder;
+  for (int i = 0; i < G - 1; i++) dorder.push_back(i);
+  std::stable_sort(dorder.begin(), dorder.end(), [&](int x, int y) { return depth[x] < depth[y]; });
+  DBuf b_dl;
+  int *dl;
+  ENSURE(b_dl, std::max<size_t>(dorder.size(), 1), &dl);
+  if (!dorder.empty()) HIPCHECK(hipMemcpy(dl, dorder.data(), dorder.size() * 4, hipMemcpyHostToDevice));
+  for (int d = 1, pos = 0; d <= max_d; d++) {
+    int n = 0;
+    while (pos + n < (int)dorder.size() && depth[dorder[pos + n]] == d) n++;
+    if (!n) continue;
+    a.level_nodes = dl + pos;
+    a.n_level = n;
+    const long work = (long)n * N;
+    hipLaunchKernelGGL((pre::proximal_level_kernel<RS, S>), dim3((work + 255) / 256), dim3(256), 0, c->stream, a);
+    HIPCHECK(hipGetLastError());
+    pos += n;
+  }
+  HIPCHECK(hipMemsetAsync(s64, 0, snn * 8, c->stream));
+  HIPCHECK(hipMemsetAsync(s32, 0, snn * 4, c->stream));
+  HIPCHECK(hipMemsetAsync(se, 0, (size_t)N * Epad * 4, c->stream));
+  const long work = (long)G * N;
+  hipLaunchKernelGGL((pre::finish_kernel<RS, S>), dim3((work + 255) / 256), dim3(256), 0, c->stream, a, statd, E,
+                     Epad, s64, se, d32, p32, s32);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  (void)hipFree(b_dl.p);
+  double *dbl;
+  ENSURE(c->b_bl, G, &dbl);
+  HIPCHECK(hipMemcpy(dbl, c->bl.data(), G * 8, hipMemcpyHostToDevice));
+  c->has_tree = true;
+  return PP_OK;
+}
+
+int gpu_precompute(pp_ctx *c, const TreeDesc &t, const char *leaf_seqs) {
+  const int G = t.n_nodes, N = t.n_sites, S = c->model.S, R = c->model.R;
+  std::vector<int> parent(G, -1), height(G, 0), depth(G, 0);
+  for (int id = 0; id < G; id++)
+    for (int k = t.child_start[id]; k < t.child_start[id + 1]; k++) {
+      const int ch = t.children[k];
+      CHECK(ch >= 0 && ch < id, PP_EINVAL, "node ids must be post-order (children before parents)");
+      CHECK(parent[ch] < 0, PP_EINVAL, "a node has two parents");
+      parent[ch] = id;
+      height[id] = std::max(height[id], height[ch] + 1);
+    }
+  for (int id = 0; id < G - 1; id++) CHECK(parent[id] >= 0, PP_EINVAL, "the root must be the last node id");
+  for (int id = G - 2; id >= 0; id--) depth[id] = depth[parent[id]] + 1;
+  std::vector<uint8_t> codes((size_t)G * N, 0);
+  for (int id = 0; id < G; id++) {
+    if (t.child_start[id + 1] != t.child_start[id]) continue;
+    for (int k = 0; k < N; k++) {
+      const char ch = leaf_seqs[(size_t)id * N + k];
+      const int code = c->model.code_of[(unsigned char)ch];
+      CHECK(code >= 0, PP_EINVAL, std::string(1, ch) + " not a known base in leaf " + std::to_string(id));
+      codes[(size_t)id * N + k] = (uint8_t)code;
+    }
+  }
+  if (S != 4) { set_error("the GPU placement path supports nucleotide models (n_states = 4)"); return PP_EINVAL; }
+  if (R == 4) return gpu_precompute_t<4, 4>(c, t, codes, parent, height, depth);
+  if (R == 1) return gpu_precompute_t<1, 4>(c, t, codes, parent, height, depth);
+  set_error("n_rates must be 1 or 4 on the GPU path");
+  return PP_EINVAL;
+}
+
 int bucket_of(int M) { return M <= 0 ? 0 : (M - 1) / 64; }  // block = 64*(bucket+1)
 
 template <typename Real>
@@ -648,16 +804,13 @@ int pp_tree_set(pp_ctx *c, int n_nodes, const int *child_start, const int *child
   int st = set_device(c);
   if (st) return st;
   TreeDesc t{n_nodes, n_sites, child_start, children, bl};
-  TreeGlvs g;
-  std::string err;
-  st = host_precompute(c->model, t, leaf_seqs, g, err);
-  if (st) { set_error(err); return st; }
+  c->has_tree = false;
   c->G = n_nodes;
   c->N = n_sites;
   c->E = n_nodes - 1;
   c->Epad = (c->E + 255) / 256 * 256;
   c->bl.assign(bl, bl + n_nodes);
-  return upload_glvs(c, g);
+  return gpu_precompute(c, t, leaf_seqs);
 }
 
 int pp_tree_upload(pp_ctx *c, int n_nodes, const double *bl, int n_sites, const double *darr,

@@ -422,3 +422,185 @@ __global__ void logsum_kernel(const double *util, const uint16_t *mask, int firs
 }
 
 }  // namespace pp
+
+// ---------------------------------------------------------------------------
+// Reference-tree precompute (Like_stree.calc_distal_and_proximal,
+// like_stree.ml:43-110; perhaps_pull_exponent gmix_model.ml:163-187;
+// Glv_arr.prep_supernodes glv_arr.ml:46-56), level-synchronous: one launch
+// per post-order height (distal) and per pre-order depth (proximal), one
+// lane per (node, site).  GLV rows are [node][site][R*S] FP64; P tables are
+// FP64 [node][R][S][S] from the host's Diagd.multi_exp.  The arithmetic is
+// the stubs': products in list order, sequential gemmish sums, exact
+// power-of-two exponent pulls.
+// ---------------------------------------------------------------------------
+namespace pre {
+
+template <int RS, int S>
+__device__ __forceinline__ void pull(double *v, int &e, int min_twoexp) {
+  constexpr int R = RS / S;
+  int mx_te = INT_MIN;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    double mx = v[r * S];
+#pragma unroll
+    for (int k = 1; k < S; k++)
+      if (v[r * S + k] > mx) mx = v[r * S + k];
+    int te;
+    (void)frexp(mx, &te);
+    if (te > mx_te) mx_te = te;
+  }
+  if (mx_te < min_twoexp) {
+    const double f = ldexp(0.5, -mx_te + 1);
+#pragma unroll
+    for (int t = 0; t < RS; t++) v[t] *= f;
+    e += mx_te;
+  }
+}
+
+template <int RS, int S>
+__device__ __forceinline__ void evolve_row(const double *P, const double *src, double *dst) {
+  constexpr int R = RS / S;
+#pragma unroll
+  for (int r = 0; r < R; r++)
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+      double acc = 0;
+#pragma unroll
+      for (int j = 0; j < S; j++) acc = acc + P[(r * S + i) * S + j] * src[r * S + j];
+      dst[r * S + i] = acc;
+    }
+}
+
+struct TreeArgs {
+  int N, n_nodes;
+  const int *level_nodes;   // node ids of this launch
+  int n_level;
+  const int *child_start, *children, *parent;
+  const double *P, *Phalf;  // [G][R][S][S]
+  const uint8_t *leaf_codes;  // [G][N]
+  const double *lv;           // [n_codes][S]
+  double *darr, *parr, *evd;  // [G][N][RS]
+  int *darr_e, *parr_e, *evd_e;
+};
+
+// post-order: distal = product of the children's evolved distals (or the
+// leaf's LV), pulled at 2^-50, then evolv_dist = P(bl) * distal
+template <int RS, int S>
+__global__ void distal_level_kernel(TreeArgs a) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)a.n_level * a.N) return;
+  const int site = (int)(t % a.N);
+  const int node = a.level_nodes[t / a.N];
+  const size_t row = (size_t)node * a.N + site;
+  double v[RS];
+  int e = 0;
+  const int c0 = a.child_start[node], c1 = a.child_start[node + 1];
+  if (c1 == c0) {
+    const int code = a.leaf_codes[row];
+#pragma unroll
+    for (int r = 0; r < RS / S; r++)
+#pragma unroll
+      for (int k = 0; k < S; k++) v[r * S + k] = a.lv[code * S + k];
+  } else {
+    const size_t r0 = (size_t)a.children[c0] * a.N + site;
+#pragma unroll
+    for (int k = 0; k < RS; k++) v[k] = a.evd[r0 * RS + k];
+    e = a.evd_e[r0];
+    for (int c = c0 + 1; c < c1; c++) {
+      const size_t rc = (size_t)a.children[c] * a.N + site;
+#pragma unroll
+      for (int k = 0; k < RS; k++) v[k] = v[k] * a.evd[rc * RS + k];
+      e += a.evd_e[rc];
+    }
+    pull<RS, S>(v, e, -50);
+  }
+#pragma unroll
+  for (int k = 0; k < RS; k++) a.darr[row * RS + k] = v[k];
+  a.darr_e[row] = e;
+  double w[RS];
+  evolve_row<RS, S>(a.P + (size_t)node * RS * S, v, w);
+#pragma unroll
+  for (int k = 0; k < RS; k++) a.evd[row * RS + k] = w[k];
+  a.evd_e[row] = e;
+}
+
+// pre-order: proximal of a child = P(bl_parent) * proximal(parent) times the
+// evolved distals of its siblings in order, pulled at 2^-50
+template <int RS, int S>
+__global__ void proximal_level_kernel(TreeArgs a) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)a.n_level * a.N) return;
+  const int site = (int)(t % a.N);
+  const int child = a.level_nodes[t / a.N];
+  const int par = a.parent[child];
+  const size_t prow = (size_t)par * a.N + site;
+  double src[RS], v[RS];
+#pragma unroll
+  for (int k = 0; k < RS; k++) src[k] = a.parr[prow * RS + k];
+  int e = a.parr_e[prow];
+  evolve_row<RS, S>(a.P + (size_t)par * RS * S, src, v);
+  for (int c = a.child_start[par]; c < a.child_start[par + 1]; c++) {
+    const int sib = a.children[c];
+    if (sib == child) continue;
+    const size_t rs = (size_t)sib * a.N + site;
+#pragma unroll
+    for (int k = 0; k < RS; k++) v[k] = v[k] * a.evd[rs * RS + k];
+    e += a.evd_e[rs];
+  }
+  pull<RS, S>(v, e, -50);
+  const size_t row = (size_t)child * a.N + site;
+#pragma unroll
+  for (int k = 0; k < RS; k++) a.parr[row * RS + k] = v[k];
+  a.parr_e[row] = e;
+}
+
+__global__ void fill_unit_kernel(double *row, int *e, int N, int RS) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < (long)N * RS; t += (long)gridDim.x * blockDim.x) {
+    row[t] = 1.0;
+    if (t < N) e[t] = 0;
+  }
+}
+
+// pull both arrays at 2^-10 (pplacer_run.ml:331-336), then the supernodes
+// pi * P(bl/2) darr * P(bl/2) parr (glv_arr.ml:46-56), written site-major
+// for stage 1, with FP32 copies of everything
+template <int RS, int S>
+__global__ void finish_kernel(TreeArgs a, const double *statd, int E, int Epad, double *sn64,
+                              int *sn_e, float *d32, float *p32, float *sn32) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)a.n_nodes * a.N) return;
+  const int site = (int)(t % a.N);
+  const int node = (int)(t / a.N);
+  const size_t row = (size_t)node * a.N + site;
+  double d[RS], p[RS];
+#pragma unroll
+  for (int k = 0; k < RS; k++) { d[k] = a.darr[row * RS + k]; p[k] = a.parr[row * RS + k]; }
+  int ed = a.darr_e[row], ep = a.parr_e[row];
+  pull<RS, S>(d, ed, -10);
+  pull<RS, S>(p, ep, -10);
+#pragma unroll
+  for (int k = 0; k < RS; k++) {
+    a.darr[row * RS + k] = d[k];
+    a.parr[row * RS + k] = p[k];
+    d32[row * RS + k] = (float)d[k];
+    p32[row * RS + k] = (float)p[k];
+  }
+  a.darr_e[row] = ed;
+  a.parr_e[row] = ep;
+  if (node < E) {
+    double ud[RS], up[RS];
+    const double *Ph = a.Phalf + (size_t)node * RS * S;
+    evolve_row<RS, S>(Ph, d, ud);
+    evolve_row<RS, S>(Ph, p, up);
+    const size_t srow = (size_t)site * Epad + node;
+#pragma unroll
+    for (int k = 0; k < RS; k++) {
+      const double s = (statd[k % S] * ud[k]) * up[k];
+      sn64[srow * RS + k] = s;
+      sn32[srow * RS + k] = (float)s;
+    }
+    sn_e[srow] = ed + ep;
+  }
+}
+
+}  // namespace pre

@@ -20,7 +20,5 @@ struct TreeGlvs {
   std::vector<int64_t> darr_e, parr_e, snodes_e;
 };
 
-int host_precompute(const HostModel &m, const TreeDesc &t, const char *leaf_seqs,
-                    TreeGlvs &out, std::string &err);
 
 }  // namespace pp
