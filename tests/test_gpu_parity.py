@@ -212,3 +212,54 @@ def test_place_edge_cases(gtr50):
     # keep_at_most = 1 returns the single best placement with ratio 1
     got, status, _ = pl.place(queries[:4], prefs=default_prefs(keep_at_most=1))
     assert all(len(g) == 1 and g[0].like_weight_ratio == 1.0 for g in got)
+
+
+# ---- end to end on the reference's own test data --------------------------------
+
+def test_run_placements_on_reference_fixture(tmp_path, oracle):
+    """Queries cut from reference sequences of tests/data/like/gtr (HIV, 20
+    taxa x 216 columns, PhyML GTR+G4) place on their own leaf edge, agree
+    with the oracle, and are written as jplace."""
+    import json as _json
+
+    from pplacer_amd import jplace, refio, run
+    from tests.helpers import GOLDEN, cut
+    _print_case("run_placements on like/gtr fixture")
+    d = GOLDEN + "/like/gtr/"
+    tree = refio.read_newick(d + "hiv.phy_phyml_tree.txt")
+    aln = refio.read_fasta(d + "hiv.fasta")
+    pm = refio.read_phylo_model(d + "phylo_model.jplace")
+    leaves = tree.leaves()
+    queries = []
+    for k, i in enumerate(leaves[:8]):
+        seq = dict(aln)[tree.names[i]]
+        a, b = 20 + 7 * k, 150 + 5 * k
+        queries.append(("q_%s" % tree.names[i], "-" * a + seq[a:b] + "-" * (len(seq) - b)))
+    # a duplicate query is placed once and reported under both names
+    queries.append(("dup", queries[0][1]))
+    pqueries, status = run.run_placements(tree, aln, queries, pm)
+    assert np.all(status == 0)
+    assert len(pqueries) == 8
+    names = {nl[0][0]: (nl, recs) for nl, recs in pqueries}
+    assert [n for n, _ in names["q_" + tree.names[leaves[0]]][0]] == ["q_" + tree.names[leaves[0]], "dup"]
+    # oracle on the same inputs
+    ref_seqs = [s for _, s in aln]
+    model = oracle.Model(refio.gtr_exchangeabilities(pm.subs_rates), oracle.emper_freq(4, ref_seqs),
+                         oracle.discrete_gamma(pm.n_cats, pm.alpha))
+    st, mask, keep, n = oracle.premask(4, ref_seqs, [s for _, s in queries])
+    leaf = {i: cut(dict(aln)[tree.names[i]], mask) for i in leaves}
+    pre = oracle.Precompute(model, tree, leaf, n)
+    pre.pull_and_supernodes()
+    core = oracle.Core(pre)
+    for k, i in enumerate(leaves[:8]):
+        nl, recs = names["q_" + tree.names[i]]
+        ost, want, _ = core.place(cut(queries[k][1], mask))
+        assert recs[0].edge_num == want[0][0]
+        assert abs(recs[0].likelihood - want[0][2]) <= 1e-9 * abs(want[0][2])
+        # a fragment of a leaf's own sequence lands on that leaf's edge with a
+        # short pendant
+        assert recs[0].edge_num == i or recs[0].pendant_length < 0.02
+    out = tmp_path / "out.jplace"
+    jplace.write(str(out), tree, pqueries)
+    obj = _json.loads(out.read_text())
+    assert len(obj["placements"]) == 8 and obj["fields"] == jplace.FIELDS
