@@ -93,6 +93,7 @@ struct pp_ctx {
   int G = 0, N = 0, E = 0, Epad = 0;
   std::vector<double> bl;
   DBuf b_bl, b_d64, b_p64, b_d32, b_p32, b_de, b_pe, b_s64, b_s32, b_se;
+  DBuf b_de32, b_pe32, b_se32;  // exponents of the FP32 copies
   // resident queries (sorted by first informative column)
   int Q = 0;
   std::vector<int> order;     // sorted position -> caller's index
@@ -149,19 +150,17 @@ void to_device_layout(const TreeGlvs &g, int G, int N, int R, int S, int Epad,
         }
 }
 
+int make_fp32_copies(pp_ctx *c);
+
 int upload_glvs(pp_ctx *c, const TreeGlvs &g) {
   const int G = c->G, N = c->N, R = c->model.R, S = c->model.S, Epad = c->Epad;
   if (S != 4) { set_error("the GPU placement path supports nucleotide models (n_states = 4)"); return PP_EINVAL; }
   const size_t glv = (size_t)G * N * R * S, snn = (size_t)N * Epad * R * S;
   double *d64, *p64, *s64;
-  float *d32, *p32, *s32;
   int *de, *pe, *se;
   ENSURE(c->b_d64, glv, &d64);
   ENSURE(c->b_p64, glv, &p64);
   ENSURE(c->b_s64, snn, &s64);
-  ENSURE(c->b_d32, glv, &d32);
-  ENSURE(c->b_p32, glv, &p32);
-  ENSURE(c->b_s32, snn, &s32);
   ENSURE(c->b_de, (size_t)G * N, &de);
   ENSURE(c->b_pe, (size_t)G * N, &pe);
   ENSURE(c->b_se, (size_t)N * Epad, &se);
@@ -172,13 +171,6 @@ int upload_glvs(pp_ctx *c, const TreeGlvs &g) {
     HIPCHECK(hipMemcpy(p64, b.data(), b.size() * 8, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(s64, s.data(), s.size() * 8, hipMemcpyHostToDevice));
   }
-  {
-    std::vector<float> a, b, s;
-    to_device_layout<float>(g, G, N, R, S, Epad, a, b, s);
-    HIPCHECK(hipMemcpy(d32, a.data(), a.size() * 4, hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(p32, b.data(), b.size() * 4, hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(s32, s.data(), s.size() * 4, hipMemcpyHostToDevice));
-  }
   std::vector<int> ed((size_t)G * N), ep((size_t)G * N), es((size_t)N * Epad, 0);
   for (size_t i = 0; i < ed.size(); i++) { ed[i] = (int)g.darr_e[i]; ep[i] = (int)g.parr_e[i]; }
   for (int i = 0; i < G - 1; i++)
@@ -186,10 +178,35 @@ int upload_glvs(pp_ctx *c, const TreeGlvs &g) {
   HIPCHECK(hipMemcpy(de, ed.data(), ed.size() * 4, hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(pe, ep.data(), ep.size() * 4, hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(se, es.data(), es.size() * 4, hipMemcpyHostToDevice));
+  int st = make_fp32_copies(c);
+  if (st) return st;
   double *dbl;
   ENSURE(c->b_bl, G, &dbl);
   HIPCHECK(hipMemcpy(dbl, c->bl.data(), G * 8, hipMemcpyHostToDevice));
   c->has_tree = true;
+  return PP_OK;
+}
+
+// FP32 copies of the resident FP64 GLVs with their own exponents
+int make_fp32_copies(pp_ctx *c) {
+  const int G = c->G, N = c->N, RS = c->model.R * c->model.S, Epad = c->Epad;
+  float *d32, *p32, *s32;
+  int *de32, *pe32, *se32;
+  ENSURE(c->b_d32, (size_t)G * N * RS, &d32);
+  ENSURE(c->b_p32, (size_t)G * N * RS, &p32);
+  ENSURE(c->b_s32, (size_t)N * Epad * RS, &s32);
+  ENSURE(c->b_de32, (size_t)G * N, &de32);
+  ENSURE(c->b_pe32, (size_t)G * N, &pe32);
+  ENSURE(c->b_se32, (size_t)N * Epad, &se32);
+  const dim3 grid(4096), block(256);
+  hipLaunchKernelGGL(fp32_rescale_kernel, grid, block, 0, c->stream, (const double *)c->b_d64.p,
+                     (const int *)c->b_de.p, d32, de32, (long)G * N, RS);
+  hipLaunchKernelGGL(fp32_rescale_kernel, grid, block, 0, c->stream, (const double *)c->b_p64.p,
+                     (const int *)c->b_pe.p, p32, pe32, (long)G * N, RS);
+  hipLaunchKernelGGL(fp32_rescale_kernel, grid, block, 0, c->stream, (const double *)c->b_s64.p,
+                     (const int *)c->b_se.p, s32, se32, (long)N * Epad, RS);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(c->stream));
   return PP_OK;
 }
 
@@ -203,15 +220,11 @@ int gpu_precompute_t(pp_ctx *c, const TreeDesc &t, const std::vector<uint8_t> &c
   const int G = c->G, N = c->N, E = c->E, Epad = c->Epad;
   const size_t glv = (size_t)G * N * RS, snn = (size_t)N * Epad * RS;
   double *d64, *p64, *s64, *P, *Ph, *lv, *statd;
-  float *d32, *p32, *s32;
   int *de, *pe, *se, *cs, *ch, *par, *lvl;
   uint8_t *lc;
   ENSURE(c->b_d64, glv, &d64);
   ENSURE(c->b_p64, glv, &p64);
   ENSURE(c->b_s64, snn, &s64);
-  ENSURE(c->b_d32, glv, &d32);
-  ENSURE(c->b_p32, glv, &p32);
-  ENSURE(c->b_s32, snn, &s32);
   ENSURE(c->b_de, (size_t)G * N, &de);
   ENSURE(c->b_pe, (size_t)G * N, &pe);
   ENSURE(c->b_se, (size_t)N * Epad, &se);
@@ -304,14 +317,15 @@ int gpu_precompute_t(pp_ctx *c, const TreeDesc &t, const std::vector<uint8_t> &c
     pos += n;
   }
   HIPCHECK(hipMemsetAsync(s64, 0, snn * 8, c->stream));
-  HIPCHECK(hipMemsetAsync(s32, 0, snn * 4, c->stream));
   HIPCHECK(hipMemsetAsync(se, 0, (size_t)N * Epad * 4, c->stream));
   const long work = (long)G * N;
   hipLaunchKernelGGL((pre::finish_kernel<RS, S>), dim3((work + 255) / 256), dim3(256), 0, c->stream, a, statd, E,
-                     Epad, s64, se, d32, p32, s32);
+                     Epad, s64, se);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(c->stream));
   (void)hipFree(b_dl.p);
+  int st = make_fp32_copies(c);
+  if (st) return st;
   double *dbl;
   ENSURE(c->b_bl, G, &dbl);
   HIPCHECK(hipMemcpy(dbl, c->bl.data(), G * 8, hipMemcpyHostToDevice));
@@ -395,7 +409,7 @@ int launch_stage1(pp_ctx *c, int q0, int nq, double start_pend, double *scores,
   a.Epad = c->Epad;
   a.n_codes = c->model.n_codes;
   a.n_inf = c->model.n_inf;
-  a.sn_e = reinterpret_cast<const int *>(c->b_se.p);
+  a.sn_e = reinterpret_cast<const int *>(std::is_same<Real, double>::value ? c->b_se.p : c->b_se32.p);
   a.tcodes = d_tc;
   a.tile_lo = d_tlo;
   a.tile_hi = d_thi;
@@ -590,8 +604,8 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
       p.counter = d_counter + b;
       p.darr = fp32 ? c->b_d32.p : c->b_d64.p;
       p.parr = fp32 ? c->b_p32.p : c->b_p64.p;
-      p.darr_e = reinterpret_cast<const int *>(c->b_de.p);
-      p.parr_e = reinterpret_cast<const int *>(c->b_pe.p);
+      p.darr_e = reinterpret_cast<const int *>(fp32 ? c->b_de32.p : c->b_de.p);
+      p.parr_e = reinterpret_cast<const int *>(fp32 ? c->b_pe32.p : c->b_pe.p);
       p.bl = reinterpret_cast<const double *>(c->b_bl.p);
       p.N = N;
       p.E = E;
@@ -758,7 +772,7 @@ int pp_ctx_destroy(pp_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   DBuf *bufs[] = {&c->b_model, &c->b_bl, &c->b_d64, &c->b_p64, &c->b_d32, &c->b_p32, &c->b_de,
-                  &c->b_pe, &c->b_s64, &c->b_s32, &c->b_se, &c->b_codes, &c->b_qM, &c->b_tcodes,
+                  &c->b_pe, &c->b_s64, &c->b_s32, &c->b_se, &c->b_de32, &c->b_pe32, &c->b_se32, &c->b_codes, &c->b_qM, &c->b_tcodes,
                   &c->b_tile_lo, &c->b_tile_hi, &c->b_tile_q, &c->b_ttab, &c->b_scores, &c->b_res,
                   &c->b_nres, &c->b_status, &c->b_kept, &c->b_nkeep, &c->b_out, &c->b_nout,
                   &c->b_qlist, &c->b_counter, &c->b_stats};
@@ -1014,8 +1028,8 @@ int pp_optimize_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
     const bool fp32 = precision == PP_FP32;
     o.darr = fp32 ? c->b_d32.p : c->b_d64.p;
     o.parr = fp32 ? c->b_p32.p : c->b_p64.p;
-    o.darr_e = reinterpret_cast<const int *>(c->b_de.p);
-    o.parr_e = reinterpret_cast<const int *>(c->b_pe.p);
+    o.darr_e = reinterpret_cast<const int *>(fp32 ? c->b_de32.p : c->b_de.p);
+    o.parr_e = reinterpret_cast<const int *>(fp32 ? c->b_pe32.p : c->b_pe.p);
     o.bl = reinterpret_cast<const double *>(c->b_bl.p);
     o.tol = tolerance;
     o.max_pend = max_pend;

@@ -566,7 +566,7 @@ __global__ void fill_unit_kernel(double *row, int *e, int N, int RS) {
 // for stage 1, with FP32 copies of everything
 template <int RS, int S>
 __global__ void finish_kernel(TreeArgs a, const double *statd, int E, int Epad, double *sn64,
-                              int *sn_e, float *d32, float *p32, float *sn32) {
+                              int *sn_e) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= (long)a.n_nodes * a.N) return;
   const int site = (int)(t % a.N);
@@ -582,8 +582,6 @@ __global__ void finish_kernel(TreeArgs a, const double *statd, int E, int Epad, 
   for (int k = 0; k < RS; k++) {
     a.darr[row * RS + k] = d[k];
     a.parr[row * RS + k] = p[k];
-    d32[row * RS + k] = (float)d[k];
-    p32[row * RS + k] = (float)p[k];
   }
   a.darr_e[row] = ed;
   a.parr_e[row] = ep;
@@ -594,13 +592,30 @@ __global__ void finish_kernel(TreeArgs a, const double *statd, int E, int Epad, 
     evolve_row<RS, S>(Ph, p, up);
     const size_t srow = (size_t)site * Epad + node;
 #pragma unroll
-    for (int k = 0; k < RS; k++) {
-      const double s = (statd[k % S] * ud[k]) * up[k];
-      sn64[srow * RS + k] = s;
-      sn32[srow * RS + k] = (float)s;
-    }
+    for (int k = 0; k < RS; k++) sn64[srow * RS + k] = (statd[k % S] * ud[k]) * up[k];
     sn_e[srow] = ed + ep;
   }
 }
 
 }  // namespace pre
+
+// FP32 copy of a GLV array with its own per-site power-of-two rescaling: each
+// row of R*S values is scaled by 2^-k, k the frexp exponent of the row's
+// largest value, and k is added to the row's exponent.  The FP64 arrays keep
+// the reference's pulls (gmix_model.ml:163-187), which stop pulling a site
+// once one rate's vector is exactly zero (frexp 0 = exponent 0); their values
+// can then sit far below FP32's range, hence the separate scaling.
+__global__ void fp32_rescale_kernel(const double *src, const int *e, float *dst, int *e32,
+                                    long rows, int RS) {
+  for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < rows;
+       r += (long)gridDim.x * blockDim.x) {
+    const double *v = src + r * RS;
+    double mx = 0.0;
+    for (int k = 0; k < RS; k++) mx = v[k] > mx ? v[k] : mx;
+    int te = 0;
+    if (mx > 0.0) (void)frexp(mx, &te);
+    const double sc = ldexp(1.0, -te);
+    for (int k = 0; k < RS; k++) dst[r * RS + k] = (float)(v[k] * sc);
+    e32[r] = e[r] + te;
+  }
+}
