@@ -16,13 +16,32 @@ namespace pp {
 // reuses each row for the TQ queries of the tile, whose characters sit in
 // tcodes[tile][site][TQ] (wave-uniform loads).  Queries are tiled in order
 // of their first informative column, so a tile's columns overlap.
-// The per-site util is the stub's sequential (rate, state) sum and the logs
-// are added in site order, exactly as ten_masked_logdot_c does.
+// The per-site util is the stub's sequential (rate, state) sum; its log is
+// taken through frexp mantissas folded over kLogGroup sites (below).
 // ---------------------------------------------------------------------------
+template <typename Real> struct Frexp;
+template <> struct Frexp<double> {
+  static __device__ __forceinline__ double mant(double x) { return __builtin_amdgcn_frexp_mant(x); }
+  static __device__ __forceinline__ int expo(double x) { return __builtin_amdgcn_frexp_exp(x); }
+  static __device__ __forceinline__ double logd(double x) { return log(x); }
+};
+template <> struct Frexp<float> {
+  static __device__ __forceinline__ float mant(float x) { return __builtin_amdgcn_frexp_mantf(x); }
+  static __device__ __forceinline__ int expo(float x) { return __builtin_amdgcn_frexp_expf(x); }
+  static __device__ __forceinline__ double logd(float x) { return (double)__logf(x); }
+};
+
+// Sites whose utils are folded into one log: the product of kLogGroup
+// frexp mantissas stays >= 2^-kLogGroup, and the binary exponents are summed
+// as integers, so log(util_1 ... util_k) = log(prod mantissas) + (sum exps) ln 2
+// has no underflow and needs one log per kLogGroup sites.
+constexpr int kLogGroup = 8;
+
 template <typename Real, int R, int TQ>
 __global__ __launch_bounds__(256) void stage1_kernel(Stage1Args a) {
   constexpr int RS = R * 4;
   using Ops = RealOps<Real>;
+  using F = Frexp<Real>;
   __shared__ Real tt[kMaxCodes * RS];
   for (int t = threadIdx.x; t < a.n_codes * RS; t += blockDim.x) tt[t] = (Real)a.ttab[t];
   __syncthreads();
@@ -31,9 +50,11 @@ __global__ __launch_bounds__(256) void stage1_kernel(Stage1Args a) {
   const int lo = a.tile_lo[tile], hi = a.tile_hi[tile];
   const int n_inf = a.n_inf;
   double ll[TQ];
-  int es[TQ];
+  Real pm[TQ];   // running product of util mantissas
+  int pe[TQ];    // util binary exponents + supernode exponents
+  int cnt[TQ];   // sites folded into pm
 #pragma unroll
-  for (int j = 0; j < TQ; j++) { ll[j] = 0.0; es[j] = 0; }
+  for (int j = 0; j < TQ; j++) { ll[j] = 0.0; pm[j] = 1; pe[j] = 0; cnt[j] = 0; }
   const Real *snp = reinterpret_cast<const Real *>(a.sn) + ((size_t)lo * a.Epad + e) * RS;
   const int *sep = a.sn_e + (size_t)lo * a.Epad + e;
   const uint8_t *cp = a.tcodes + ((size_t)tile * a.N + lo) * TQ;
@@ -51,8 +72,13 @@ __global__ __launch_bounds__(256) void stage1_kernel(Stage1Args a) {
         Real util = 0;
 #pragma unroll
         for (int t = 0; t < RS; t++) util = Ops::madd(util, z[t], s[t]);
-        ll[j] += Ops::logd(util);
-        es[j] += ex;
+        pm[j] *= F::mant(util);
+        pe[j] += F::expo(util) + ex;
+        if (++cnt[j] == kLogGroup) {
+          ll[j] += F::logd(pm[j]);
+          pm[j] = 1;
+          cnt[j] = 0;
+        }
       }
     }
     snp += sstride;
@@ -63,9 +89,10 @@ __global__ __launch_bounds__(256) void stage1_kernel(Stage1Args a) {
 #pragma unroll
     for (int j = 0; j < TQ; j++) {
       const int q = a.tile_q[tile * TQ + j];
-      if (q >= 0)
-        a.scores[(size_t)q * a.E + e] =
-            (ll[j] - (double)a.qM[q] * a.log_R) + PP_LOG2 * (0.0 + (double)es[j]);
+      if (q >= 0) {
+        const double l = ll[j] + F::logd(pm[j]);
+        a.scores[(size_t)q * a.E + e] = (l - (double)a.qM[q] * a.log_R) + PP_LOG2 * (double)pe[j];
+      }
     }
   }
 }
