@@ -171,7 +171,7 @@ typedef struct {
   int64_t n_like_calls;  /* log_like3 evaluations (Three_tax.n_like_calls) */
   int64_t n_pitched;     /* Brent-optimised (query, edge) pairs */
   /* per-kernel device time (HIP events on the library's stream) */
-  double t_stage1_ms;    /* stage-1 scoring kernel */
+  double t_stage1_ms;    /* stage-1 scoring kernels (tile codes + scores) */
   double t_pitch_ms;     /* baseball + Brent kernels */
   double t_refine_ms;    /* keep + final refinement + finalize kernels */
   /* algorithmic work: sums over evaluations of the masked length M */
@@ -179,6 +179,7 @@ typedef struct {
   int64_t n_site_probes_pend; /* sum over pendant-only probes of M (stage 2) */
   int64_t n_site_probes_dist; /* sum over probes that re-evolve prox/dist of M */
   int64_t n_launches;         /* kernel launches issued */
+  double t_table_ms;          /* stage-1 log-util table build (part of the ranking time) */
 } pp_stats;
 
 /* Stage 1 for a batch: scores[q][e] = Glv.masked_logdot of the query (full

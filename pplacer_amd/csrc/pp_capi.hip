@@ -45,6 +45,8 @@ namespace {
 
 constexpr int kTQ = 16;           // queries per stage-1 tile
 constexpr int kChunk = 32768;     // queries per device batch
+constexpr int kBuckets = 15;      // masked-length classes of 64 sites (M <= 960)
+constexpr int kMaxM = 64 * kBuckets;
 
 struct DBuf {
   void *p = nullptr;
@@ -92,8 +94,10 @@ struct pp_ctx {
   bool has_tree = false;
   int G = 0, N = 0, E = 0, Epad = 0;
   std::vector<double> bl;
-  DBuf b_bl, b_d64, b_p64, b_d32, b_p32, b_de, b_pe, b_s64, b_s32, b_se;
-  DBuf b_de32, b_pe32, b_se32;  // exponents of the FP32 copies
+  DBuf b_bl, b_d64, b_p64, b_d32, b_p32, b_de, b_pe, b_s64, b_se;
+  DBuf b_de32, b_pe32;  // exponents of the FP32 copies
+  DBuf b_lt;            // stage-1 log-util table [N][n_inf+1][Epad]
+  std::vector<double> h_ttab;
   // resident queries (sorted by first informative column)
   int Q = 0;
   std::vector<int> order;     // sorted position -> caller's index
@@ -189,22 +193,18 @@ int upload_glvs(pp_ctx *c, const TreeGlvs &g) {
 
 // FP32 copies of the resident FP64 GLVs with their own exponents
 int make_fp32_copies(pp_ctx *c) {
-  const int G = c->G, N = c->N, RS = c->model.R * c->model.S, Epad = c->Epad;
-  float *d32, *p32, *s32;
-  int *de32, *pe32, *se32;
+  const int G = c->G, N = c->N, RS = c->model.R * c->model.S;
+  float *d32, *p32;
+  int *de32, *pe32;
   ENSURE(c->b_d32, (size_t)G * N * RS, &d32);
   ENSURE(c->b_p32, (size_t)G * N * RS, &p32);
-  ENSURE(c->b_s32, (size_t)N * Epad * RS, &s32);
   ENSURE(c->b_de32, (size_t)G * N, &de32);
   ENSURE(c->b_pe32, (size_t)G * N, &pe32);
-  ENSURE(c->b_se32, (size_t)N * Epad, &se32);
   const dim3 grid(4096), block(256);
   hipLaunchKernelGGL(fp32_rescale_kernel, grid, block, 0, c->stream, (const double *)c->b_d64.p,
                      (const int *)c->b_de.p, d32, de32, (long)G * N, RS);
   hipLaunchKernelGGL(fp32_rescale_kernel, grid, block, 0, c->stream, (const double *)c->b_p64.p,
                      (const int *)c->b_pe.p, p32, pe32, (long)G * N, RS);
-  hipLaunchKernelGGL(fp32_rescale_kernel, grid, block, 0, c->stream, (const double *)c->b_s64.p,
-                     (const int *)c->b_se.p, s32, se32, (long)N * Epad, RS);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(c->stream));
   return PP_OK;
@@ -363,11 +363,42 @@ int gpu_precompute(pp_ctx *c, const TreeDesc &t, const char *leaf_seqs) {
   return PP_EINVAL;
 }
 
-int bucket_of(int M) { return M <= 0 ? 0 : (M - 1) / 64; }  // block = 64*(bucket+1)
+int bucket_of(int M) { return M <= 0 ? 0 : (M - 1) / 64; }  // workers = bucket+1
+int block_of_bucket(int b) { return 64 * (b + 2); }           // + one master wave
 
+// The stage-1 log-util table for one start_pend (see code_log_kernel).
+// Built from the FP64 supernodes for both forms; the FP32 form stores it in
+// FP32.
 template <typename Real>
-int launch_stage1(pp_ctx *c, int q0, int nq, double start_pend, double *scores,
-                  hipEvent_t ev_start = nullptr) {
+int build_code_logs(pp_ctx *c, double start_pend) {
+  const int N = c->N, Epad = c->Epad, n_inf = c->model.n_inf, R = c->model.R;
+  c->h_ttab = query_table(c->model, start_pend);
+  double *d_ttab;
+  Real *lt;
+  ENSURE(c->b_ttab, c->h_ttab.size(), &d_ttab);
+  ENSURE(c->b_lt, (size_t)N * (n_inf + 1) * Epad, &lt);
+  HIPCHECK(hipMemcpyAsync(d_ttab, c->h_ttab.data(), c->h_ttab.size() * 8, hipMemcpyHostToDevice,
+                          c->stream));
+  CodeLogArgs a;
+  a.sn = reinterpret_cast<const double *>(c->b_s64.p);
+  a.N = N;
+  a.Epad = Epad;
+  a.n_inf = n_inf;
+  a.ttab = d_ttab;
+  a.lt64 = std::is_same<Real, double>::value ? reinterpret_cast<double *>(lt) : nullptr;
+  a.lt32 = std::is_same<Real, float>::value ? reinterpret_cast<float *>(lt) : nullptr;
+  const dim3 grid(Epad / 256, N);
+  if (R == 4) hipLaunchKernelGGL(code_log_kernel<16>, grid, dim3(256), 0, c->stream, a);
+  else if (R == 1) hipLaunchKernelGGL(code_log_kernel<4>, grid, dim3(256), 0, c->stream, a);
+  else { set_error("n_rates must be 1 or 4 on the GPU path"); return PP_EINVAL; }
+  HIPCHECK(hipGetLastError());
+  return PP_OK;
+}
+
+// Stage-1 scores of queries [q0, q0 + nq) (sorted order) into scores[nq][E];
+// build_code_logs<Real> must have run for the same start_pend.
+template <typename Real>
+int launch_stage1(pp_ctx *c, int q0, int nq, double *scores) {
   const int N = c->N, R = c->model.R;
   const int n_tiles = (nq + kTQ - 1) / kTQ;
   std::vector<int> tlo(n_tiles), thi(n_tiles), tq(n_tiles * kTQ, -1);
@@ -388,41 +419,32 @@ int launch_stage1(pp_ctx *c, int q0, int nq, double start_pend, double *scores,
   }
   int *d_tlo, *d_thi, *d_tq;
   uint8_t *d_tc;
-  double *d_ttab;
   ENSURE(c->b_tile_lo, n_tiles, &d_tlo);
   ENSURE(c->b_tile_hi, n_tiles, &d_thi);
   ENSURE(c->b_tile_q, n_tiles * kTQ, &d_tq);
   ENSURE(c->b_tcodes, (size_t)n_tiles * N * kTQ, &d_tc);
-  std::vector<double> T = query_table(c->model, start_pend);
-  ENSURE(c->b_ttab, T.size(), &d_ttab);
   HIPCHECK(hipMemcpyAsync(d_tlo, tlo.data(), n_tiles * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemcpyAsync(d_thi, thi.data(), n_tiles * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemcpyAsync(d_tq, tq.data(), n_tiles * kTQ * 4, hipMemcpyHostToDevice, c->stream));
-  HIPCHECK(hipMemcpyAsync(d_ttab, T.data(), T.size() * 8, hipMemcpyHostToDevice, c->stream));
   const uint8_t *codes = reinterpret_cast<const uint8_t *>(c->b_codes.p) + (size_t)q0 * N;
   hipLaunchKernelGGL(tile_codes_kernel, dim3(2048), dim3(256), 0, c->stream, codes, d_tq, n_tiles,
                      N, kTQ, (uint8_t)c->model.n_inf, d_tc);
   Stage1Args a;
-  a.sn = std::is_same<Real, double>::value ? c->b_s64.p : c->b_s32.p;
+  a.lt = c->b_lt.p;
   a.N = N;
   a.E = c->E;
   a.Epad = c->Epad;
-  a.n_codes = c->model.n_codes;
-  a.n_inf = c->model.n_inf;
-  a.sn_e = reinterpret_cast<const int *>(std::is_same<Real, double>::value ? c->b_se.p : c->b_se32.p);
+  a.C1 = c->model.n_inf + 1;
+  a.sn_e = reinterpret_cast<const int *>(c->b_se.p);
   a.tcodes = d_tc;
   a.tile_lo = d_tlo;
   a.tile_hi = d_thi;
   a.tile_q = d_tq;
   a.qM = reinterpret_cast<const int *>(c->b_qM.p) + q0;
-  a.ttab = d_ttab;
   a.log_R = std::log((double)(float)R);
   a.scores = scores;
   dim3 grid(c->Epad / 256, n_tiles);
-  if (ev_start) HIPCHECK(hipEventRecord(ev_start, c->stream));
-  if (R == 4) hipLaunchKernelGGL((stage1_kernel<Real, 4, kTQ>), grid, dim3(256), 0, c->stream, a);
-  else if (R == 1) hipLaunchKernelGGL((stage1_kernel<Real, 1, kTQ>), grid, dim3(256), 0, c->stream, a);
-  else { set_error("n_rates must be 1 or 4 on the GPU path"); return PP_EINVAL; }
+  hipLaunchKernelGGL((stage1_kernel<Real, kTQ>), grid, dim3(256), 0, c->stream, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
@@ -434,30 +456,41 @@ int occupancy_grid(const void *kern, int bd, size_t lds, int n_cu, int n_items) 
   return std::max(1, std::min(n_items, nb * n_cu));
 }
 
-// Blocks of up to 512 lanes get a 256-VGPR budget; longer queries (M > 512)
-// use the 1024-lane instantiation.
-template <typename Real, int R>
-int launch_pitch(pp_ctx *c, PitchArgs a, int bd) {
-  const size_t lds = Engine4<Real, R>::engine_lds_bytes(c->N) + 4 * ((c->E + 31) / 32) + 64;
-  const void *k = bd <= 512 ? reinterpret_cast<const void *>(&pitch_kernel<Real, R, 512>)
-                            : reinterpret_cast<const void *>(&pitch_kernel<Real, R, 1024>);
-  int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.nq);
-  if (bd <= 512) hipLaunchKernelGGL((pitch_kernel<Real, R, 512>), dim3(grid), dim3(bd), lds, c->stream, a);
-  else hipLaunchKernelGGL((pitch_kernel<Real, R, 1024>), dim3(grid), dim3(bd), lds, c->stream, a);
+// Master/worker stage-2 kernels: one master wave + ceil(M/64) worker waves.
+// Three launch-bound classes give the register allocator the budget of the
+// occupancy the block size allows (512 lanes: 2 waves/SIMD, 768: 3, 1024: 4).
+template <typename Real, int R, int MAXBD>
+int launch_pitch_b(pp_ctx *c, PitchArgs a, int bd) {
+  const size_t lds = MWLayout<Real, R>::bytes(c->N) + (size_t)c->E + 64;
+  const void *k = reinterpret_cast<const void *>(&pitch_mw_kernel<Real, R, MAXBD>);
+  const int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.nq);
+  hipLaunchKernelGGL((pitch_mw_kernel<Real, R, MAXBD>), dim3(grid), dim3(bd), lds, c->stream, a);
+  HIPCHECK(hipGetLastError());
+  return PP_OK;
+}
+
+template <typename Real, int R, int MAXBD>
+int launch_opt_b(pp_ctx *c, OptArgs a, int bd) {
+  const size_t lds = MWLayout<Real, R>::bytes(c->N) + 64;
+  const void *k = reinterpret_cast<const void *>(&optimize_mw_kernel<Real, R, MAXBD>);
+  const int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.n_jobs);
+  hipLaunchKernelGGL((optimize_mw_kernel<Real, R, MAXBD>), dim3(grid), dim3(bd), lds, c->stream, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
 
 template <typename Real, int R>
+int launch_pitch(pp_ctx *c, PitchArgs a, int bd) {
+  if (bd <= 512) return launch_pitch_b<Real, R, 512>(c, a, bd);
+  if (bd <= 768) return launch_pitch_b<Real, R, 768>(c, a, bd);
+  return launch_pitch_b<Real, R, 1024>(c, a, bd);
+}
+
+template <typename Real, int R>
 int launch_opt(pp_ctx *c, OptArgs a, int bd) {
-  const size_t lds = Engine4<Real, R>::engine_lds_bytes(c->N) + 64;
-  const void *k = bd <= 512 ? reinterpret_cast<const void *>(&optimize_kernel<Real, R, 512>)
-                            : reinterpret_cast<const void *>(&optimize_kernel<Real, R, 1024>);
-  int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.n_jobs);
-  if (bd <= 512) hipLaunchKernelGGL((optimize_kernel<Real, R, 512>), dim3(grid), dim3(bd), lds, c->stream, a);
-  else hipLaunchKernelGGL((optimize_kernel<Real, R, 1024>), dim3(grid), dim3(bd), lds, c->stream, a);
-  HIPCHECK(hipGetLastError());
-  return PP_OK;
+  if (bd <= 512) return launch_opt_b<Real, R, 512>(c, a, bd);
+  if (bd <= 768) return launch_opt_b<Real, R, 768>(c, a, bd);
+  return launch_opt_b<Real, R, 1024>(c, a, bd);
 }
 
 template <typename Real>
@@ -496,7 +529,7 @@ int queries_upload(pp_ctx *c, int n_queries, const char *queries) {
       if (code < n_inf) { m++; l = std::min(l, k); h = k + 1; }
     }
     if (st[q] == PP_Q_OK && m == 0) st[q] = PP_Q_NO_INFORMATIVE;
-    if (st[q] == PP_Q_OK && m > 1024) st[q] = PP_Q_RANK_EXHAUSTED;
+    if (st[q] == PP_Q_OK && m > kMaxM) st[q] = PP_Q_RANK_EXHAUSTED;
     lo[q] = m ? l : 0;
     hi[q] = m ? h : 0;
     M[q] = m;
@@ -568,31 +601,39 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   HIPCHECK(hipEventCreate(&ev2));
   HIPCHECK(hipEventCreate(&ev3));
   const bool fp32 = precision == PP_FP32;
+  float t_table = 0;
+  HIPCHECK(hipEventRecord(ev0, c->stream));
+  st = fp32 ? build_code_logs<float>(c, prefs->start_pend) : build_code_logs<double>(c, prefs->start_pend);
+  if (st) return st;
+  launches++;
+  HIPCHECK(hipEventRecord(ev1, c->stream));
+  HIPCHECK(hipEventSynchronize(ev1));
+  HIPCHECK(hipEventElapsedTime(&t_table, ev0, ev1));
   for (int q0 = 0; q0 < Q; q0 += chunk) {
     const int nq = std::min(chunk, Q - q0);
-    st = fp32 ? launch_stage1<float>(c, q0, nq, prefs->start_pend, d_scores, ev0)
-              : launch_stage1<double>(c, q0, nq, prefs->start_pend, d_scores, ev0);
+    HIPCHECK(hipEventRecord(ev0, c->stream));
+    st = fp32 ? launch_stage1<float>(c, q0, nq, d_scores) : launch_stage1<double>(c, q0, nq, d_scores);
     if (st) return st;
     launches += 2;
     HIPCHECK(hipEventRecord(ev1, c->stream));
     // bucket queries of the chunk by masked length
-    std::vector<std::vector<int>> buckets(16);
+    std::vector<std::vector<int>> buckets(kBuckets);
     for (int l = 0; l < nq; l++)
       if (c->qstat[q0 + l] == PP_Q_OK) buckets[bucket_of(c->qM[q0 + l])].push_back(l);
     std::vector<int> flat;
-    std::vector<int> boff(17, 0);
-    for (int b = 0; b < 16; b++) {
+    std::vector<int> boff(kBuckets + 1, 0);
+    for (int b = 0; b < kBuckets; b++) {
       boff[b] = (int)flat.size();
       flat.insert(flat.end(), buckets[b].begin(), buckets[b].end());
     }
-    boff[16] = (int)flat.size();
+    boff[kBuckets] = (int)flat.size();
     if (!flat.empty())
       HIPCHECK(hipMemcpyAsync(d_qlist, flat.data(), flat.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemsetAsync(d_counter, 0, 64 * 4, c->stream));
     HIPCHECK(hipEventRecord(ev1b, c->stream));
     const uint8_t *codes = reinterpret_cast<const uint8_t *>(c->b_codes.p) + (size_t)q0 * N;
     int *status = d_status + q0;
-    for (int b = 0; b < 16; b++) {
+    for (int b = 0; b < kBuckets; b++) {
       const int n = boff[b + 1] - boff[b];
       if (!n) continue;
       PitchArgs p;
@@ -621,7 +662,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
       p.status = status;
       p.n_calls = d_stats;
       p.n_pitched = d_stats + 1;
-      st = fp32 ? dispatch_pitch<float>(c, p, 64 * (b + 1)) : dispatch_pitch<double>(c, p, 64 * (b + 1));
+      st = fp32 ? dispatch_pitch<float>(c, p, block_of_bucket(b)) : dispatch_pitch<double>(c, p, block_of_bucket(b));
       if (st) return st;
       launches++;
     }
@@ -638,7 +679,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     ka.n_keep = d_nkeep;
     hipLaunchKernelGGL(keep_kernel, dim3((nq + 255) / 256), dim3(256), 0, c->stream, ka);
     HIPCHECK(hipGetLastError());
-    for (int b = 0; b < 16; b++) {
+    for (int b = 0; b < kBuckets; b++) {
       const int n = boff[b + 1] - boff[b];
       if (!n) continue;
       OptArgs o;
@@ -663,7 +704,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
       o.n_keep = d_nkeep;
       o.status = status;
       o.n_calls = d_stats;
-      st = dispatch_opt<double>(c, o, 64 * (b + 1));
+      st = dispatch_opt<double>(c, o, block_of_bucket(b));
       if (st) return st;
       launches++;
     }
@@ -693,9 +734,10 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     unsigned long long hs[4];
     HIPCHECK(hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost));
     stats->t_stage1_ms = t_s1;
+    stats->t_table_ms = t_table;
     stats->t_pitch_ms = t_pitch;
     stats->t_refine_ms = t_ref;
-    stats->t_ranking_ms = t_s1;
+    stats->t_ranking_ms = t_table + t_s1;
     stats->t_ml_ms = t_pitch + t_ref;
     stats->n_logdots = 0;
     stats->n_site_pairs = 0;
@@ -772,7 +814,7 @@ int pp_ctx_destroy(pp_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   DBuf *bufs[] = {&c->b_model, &c->b_bl, &c->b_d64, &c->b_p64, &c->b_d32, &c->b_p32, &c->b_de,
-                  &c->b_pe, &c->b_s64, &c->b_s32, &c->b_se, &c->b_de32, &c->b_pe32, &c->b_se32, &c->b_codes, &c->b_qM, &c->b_tcodes,
+                  &c->b_pe, &c->b_s64, &c->b_se, &c->b_de32, &c->b_pe32, &c->b_lt, &c->b_codes, &c->b_qM, &c->b_tcodes,
                   &c->b_tile_lo, &c->b_tile_hi, &c->b_tile_q, &c->b_ttab, &c->b_scores, &c->b_res,
                   &c->b_nres, &c->b_status, &c->b_kept, &c->b_nkeep, &c->b_out, &c->b_nout,
                   &c->b_qlist, &c->b_counter, &c->b_stats};
@@ -935,8 +977,9 @@ int pp_stage1_scores(pp_ctx *c, int n_queries, const char *queries, double start
   const int E = c->E;
   double *d;
   ENSURE(c->b_scores, (size_t)std::max(n_queries, 1) * E, &d);
-  st = precision == PP_FP32 ? launch_stage1<float>(c, 0, n_queries, start_pend, d)
-                            : launch_stage1<double>(c, 0, n_queries, start_pend, d);
+  st = precision == PP_FP32 ? build_code_logs<float>(c, start_pend) : build_code_logs<double>(c, start_pend);
+  if (st) return st;
+  st = precision == PP_FP32 ? launch_stage1<float>(c, 0, n_queries, d) : launch_stage1<double>(c, 0, n_queries, d);
   if (st) return st;
   std::vector<double> h((size_t)n_queries * E);
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1001,18 +1044,18 @@ int pp_optimize_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
   HIPCHECK(hipMemcpy(d_st, init_st.data(), n_jobs * 4, hipMemcpyHostToDevice));
   HIPCHECK(hipMemset(d_counter, 0, 64));
   HIPCHECK(hipMemset(d_stats, 0, 32));
-  std::vector<std::vector<int>> buckets(16);
+  std::vector<std::vector<int>> buckets(kBuckets);
   for (int j = 0; j < n_jobs; j++)
     if (init_st[j] == PP_Q_OK) buckets[bucket_of(c->qM[jq[j]])].push_back(j);
   std::vector<int> flat;
-  std::vector<int> boff(17, 0);
-  for (int b = 0; b < 16; b++) {
+  std::vector<int> boff(kBuckets + 1, 0);
+  for (int b = 0; b < kBuckets; b++) {
     boff[b] = (int)flat.size();
     flat.insert(flat.end(), buckets[b].begin(), buckets[b].end());
   }
-  boff[16] = (int)flat.size();
+  boff[kBuckets] = (int)flat.size();
   int *d_list = s.up(flat.data(), flat.size());
-  for (int b = 0; b < 16; b++) {
+  for (int b = 0; b < kBuckets; b++) {
     const int n = boff[b + 1] - boff[b];
     if (!n) continue;
     OptArgs o;
@@ -1044,7 +1087,7 @@ int pp_optimize_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
     o.out_calls = d_calls;
     o.out_status = d_st;
     o.n_calls = d_stats;
-    st = fp32 ? dispatch_opt<float>(c, o, 64 * (b + 1)) : dispatch_opt<double>(c, o, 64 * (b + 1));
+    st = fp32 ? dispatch_opt<float>(c, o, block_of_bucket(b)) : dispatch_opt<double>(c, o, block_of_bucket(b));
     if (st) return st;
   }
   HIPCHECK(hipStreamSynchronize(c->stream));

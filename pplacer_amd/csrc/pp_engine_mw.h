@@ -1,0 +1,461 @@
+// Master/worker three-taxon engine (nucleotide models, S = 4).
+//
+// A workgroup is one master wave (wave 0) plus W worker waves.  The master
+// runs pplacer's sequential control -- baseball (core.ml:283-320),
+// Three_tax.optimize (three_tax.ml:70-83), Minimization.brent with the GSL
+// Brent minimiser (minimization.ml:11-100) -- on wave-uniform registers, and
+// builds each probe's transition matrices and evolved pendant rows with its 64
+// lanes (Diagd.multi_exp, diagd.ml:79-100; gemmish_c, linear_c.c:29-90).  Lane
+// j of worker wave w holds masked site 64*(w-1)+j of the query: the distal
+// and proximal GLV rows of the current edge (Glv.mask_into,
+// gmix_model.ml:121-141) and the cached (pi*prox_ev)*dist_ev products of
+// ten_log_like3_c (linear_c.c:367-424).  Each command is one LDS word and two
+// barriers:
+//
+//   master: [tables, command] -> barrier A -> .............................. -> barrier B -> [sum]
+//   worker: ..................   barrier A -> [site work, wave partial sum] -> barrier B
+//
+// Control state and site data live in different waves, so the register
+// allocator sizes the kernel for max(control, data) rather than their sum;
+// that sets how many queries a CU holds at once.
+//
+// Arithmetic order is the C stubs': each evolved entry is a sequential sum
+// over j of P[i][j]*src[j]; each site's util the sequential (rate, state) sum
+// of ((pi*prox)*dist)*pend; P entries the sequential sum over k of
+// (exp(rate*bl*l_k)*x[i][k])*xit[j][k].  The FP64 form rounds every product
+// and sum separately (-ffp-contract=off); the FP32 form uses FMAs.  Per-site
+// logs are summed per wave and then in wave order (the stubs sum them
+// sequentially in site order).
+#pragma once
+
+#include <climits>
+
+#include "pp_engine.h"
+
+namespace pp {
+
+enum { CMD_EXIT = 0, CMD_QUERY = 1, CMD_GATHER = 2, CMD_PROBE = 3 };
+
+struct MWShared {
+  double x[16], l[4], xit[16], statd[4], rates[kMaxRates];
+  double lv[kMaxCodes * 4];
+  int op, new_dist, n_slots, M, q, loc;
+  int code_slot[kMaxCodes];  // character code -> pendant row of this query
+  int slot_code[kMaxCodes];
+  double part[16];           // per-worker-wave sums of site logs
+  long long ipart[16];       // per-worker-wave sums of site exponents
+};
+
+template <typename Real, int R>
+struct MWLayout {
+  static constexpr int RS = R * 4;
+  static constexpr size_t head = (sizeof(MWShared) + 15) & ~size_t(15);
+  // shared header, pendant rows, P(prox), P(dist), masked-site list
+  static __host__ __device__ size_t bytes(int N) {
+    return head + sizeof(Real) * (kMaxCodes * RS + 2 * R * 16) + sizeof(int) * (size_t)N;
+  }
+  static __device__ Real *T(char *s) { return reinterpret_cast<Real *>(s + head); }
+  static __device__ Real *Pp(char *s) { return T(s) + kMaxCodes * RS; }
+  static __device__ Real *Pd(char *s) { return Pp(s) + R * 16; }
+  static __device__ int *sidx(char *s) { return reinterpret_cast<int *>(Pd(s) + R * 16); }
+};
+
+// Wave-uniform branch on the wave index (wave 0 = master).
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+__device__ __forceinline__ void mw_copy_model(MWShared *sh, const DevModel *dm) {
+  for (int t = threadIdx.x; t < 16; t += blockDim.x) { sh->x[t] = dm->x[t]; sh->xit[t] = dm->xit[t]; }
+  for (int t = threadIdx.x; t < 4; t += blockDim.x) { sh->l[t] = dm->l[t]; sh->statd[t] = dm->statd[t]; }
+  for (int t = threadIdx.x; t < kMaxRates; t += blockDim.x) sh->rates[t] = dm->rates[t];
+  for (int t = threadIdx.x; t < dm->n_codes * 4; t += blockDim.x) sh->lv[t] = dm->lv[t];
+}
+
+// ---------------------------------------------------------------------------
+// worker waves
+// ---------------------------------------------------------------------------
+template <typename Real, int R>
+__device__ __forceinline__ void mw_worker(char *smem, int N, const uint8_t *codes, const Real *darr,
+                          const Real *parr, const int *darr_e, const int *parr_e) {
+  using L = MWLayout<Real, R>;
+  using Ops = RealOps<Real>;
+  constexpr int RS = R * 4;
+  MWShared *sh = reinterpret_cast<MWShared *>(smem);
+  const Real *sT = L::T(smem), *sPp = L::Pp(smem), *sPd = L::Pd(smem);
+  const int *sidx = L::sidx(smem);
+  const int lane = threadIdx.x & 63;
+  const int wv = wave_id() - 1;
+  const int my = wv * 64 + lane;
+  Real dO[RS], pO[RS], w[RS];
+  int site = 0, slot = 0;
+  bool active = false;
+  for (;;) {
+    __syncthreads();  // A
+    const int op = sh->op;
+    if (op == CMD_EXIT) break;
+    if (op == CMD_QUERY) {
+      active = my < sh->M;
+      if (active) {
+        site = sidx[my];
+        slot = sh->code_slot[codes[(size_t)sh->q * N + site]];
+      }
+    } else if (op == CMD_GATHER) {
+      long long es = 0;
+      if (active) {
+        const size_t row = (size_t)sh->loc * N + site;
+        const Real *d = darr + row * RS;
+        const Real *p = parr + row * RS;
+#pragma unroll
+        for (int t = 0; t < RS; t++) { dO[t] = d[t]; pO[t] = p[t]; }
+        es = (long long)darr_e[row] + (long long)parr_e[row];
+      }
+      es = wave_sum_ll(es);
+      if (lane == 0) sh->ipart[wv] = es;
+    } else {  // CMD_PROBE
+      double lsum = 0.0;
+      if (active) {
+        if (sh->new_dist) {
+#pragma unroll
+          for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+              Real xv = 0, yv = 0;
+#pragma unroll
+              for (int j = 0; j < 4; j++) {
+                xv = Ops::madd(xv, sPp[r * 16 + i * 4 + j], pO[r * 4 + j]);
+                yv = Ops::madd(yv, sPd[r * 16 + i * 4 + j], dO[r * 4 + j]);
+              }
+              w[r * 4 + i] = ((Real)sh->statd[i] * xv) * yv;
+            }
+        }
+        const Real *z = sT + slot * RS;
+        Real util = 0;
+#pragma unroll
+        for (int t = 0; t < RS; t++) util = Ops::madd(util, w[t], z[t]);
+        lsum = Ops::logd(util);
+      }
+      lsum = wave_sum(lsum);
+      if (lane == 0) sh->part[wv] = lsum;
+    }
+    __syncthreads();  // B
+  }
+}
+
+// ---------------------------------------------------------------------------
+// master wave
+// ---------------------------------------------------------------------------
+template <typename Real, int R>
+struct MWMaster {
+  using L = MWLayout<Real, R>;
+  static constexpr int RS = R * 4;
+  MWShared *sh;
+  Real *sT, *sPp, *sPd;
+  int *sidx;
+  int lane, n_workers, n_codes, n_inf;
+  double log_R;
+
+  // Three_tax state (last probe) and caches
+  int M;
+  double pend_bl, dist_bl, prox_bl;
+  double c_pend, c_dist, c_prox;
+  double esum, last_ll;
+  bool have_last;
+  long long n_calls, n_dist_probes;
+
+  __device__ __forceinline__ void init(char *smem, const DevModel *dm) {
+    sh = reinterpret_cast<MWShared *>(smem);
+    sT = L::T(smem);
+    sPp = L::Pp(smem);
+    sPd = L::Pd(smem);
+    sidx = L::sidx(smem);
+    lane = threadIdx.x & 63;
+    n_workers = (blockDim.x >> 6) - 1;
+    n_codes = dm->n_codes;
+    n_inf = dm->n_inf;
+    log_R = dm->log_R;
+    c_pend = c_dist = c_prox = __builtin_nan("");
+  }
+
+  __device__ __forceinline__ void command(int op) {
+    if (lane == 0) sh->op = op;
+    __syncthreads();  // A
+    if (op == CMD_EXIT) return;
+    __syncthreads();  // B
+  }
+
+  // mask_reindex of the query's informative sites (core.ml:25-28,146-156),
+  // the set of characters present (their evolved rows are all a probe
+  // needs), then CMD_QUERY.  Returns M.
+  __device__ __forceinline__ int load_query(const uint8_t *codes, int N, int q) {
+    const int chunk = (N + 63) / 64;
+    const int lo = lane * chunk, hi = min(N, lo + chunk);
+    int cnt = 0;
+    unsigned present = 0;
+    for (int k = lo; k < hi; k++) {
+      const int c = codes[k];
+      if (c < n_inf) { cnt++; present |= 1u << c; }
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+      const int o = __shfl_up(incl, m, 64);
+      if (lane >= m) incl += o;
+    }
+    const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) present |= __shfl_xor(present, m, 64);
+    int pos = incl - cnt;
+    for (int k = lo; k < hi; k++)
+      if (codes[k] < n_inf) sidx[pos++] = k;
+    if (lane == 0) {
+      int ns = 0;
+      for (int c = 0; c < n_codes; c++) {
+        sh->code_slot[c] = (present >> c) & 1u ? ns : 0;
+        if ((present >> c) & 1u) sh->slot_code[ns++] = c;
+      }
+      sh->n_slots = ns;
+      sh->M = total;
+      sh->q = q;
+    }
+    M = total;
+    c_pend = __builtin_nan("");  // the pendant rows are per query
+    if (total > 0 && total <= 64 * n_workers) command(CMD_QUERY);
+    return total;
+  }
+
+  // Glv.mask_into of darr[loc] / parr[loc] and their exponent totals.
+  __device__ __forceinline__ void gather(int loc) {
+    if (lane == 0) sh->loc = loc;
+    command(CMD_GATHER);
+    long long es = 0;
+    for (int w = 0; w < n_workers; w++) es += sh->ipart[w];
+    esum = (double)es;
+    c_dist = c_prox = __builtin_nan("");
+    have_last = false;
+  }
+
+  // Diagd.multi_exp for the changed branch lengths and the evolved pendant
+  // rows of the characters present, with the master's 64 lanes.
+  __device__ __forceinline__ void tables(bool new_pend, bool new_dist) {
+    if (new_dist) {
+      // exp(rate_r * bl * l_k) for prox (u < 4R) and dist (4R <= u < 8R)
+      double ex = 0.0;
+      if (lane < 8 * R) {
+        const int which = lane / (4 * R), r = (lane / 4) % R, k = lane % 4;
+        ex = exp((sh->rates[r] * (which ? dist_bl : prox_bl)) * sh->l[k]);
+      }
+#pragma unroll
+      for (int base = 0; base < 2 * R * 16; base += 64) {
+        const int e = base + lane;
+        const int which = e / (R * 16), rem = e % (R * 16);
+        const int r = rem / 16, i = (rem / 4) % 4, j = rem % 4;
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const double exk = __shfl(ex, (which * R + r) * 4 + k, 64);
+          acc = acc + (exk * sh->x[i * 4 + k]) * sh->xit[j * 4 + k];
+        }
+        if (e < 2 * R * 16) (which ? sPd : sPp)[rem] = (Real)acc;
+      }
+    }
+    if (new_pend) {
+      double ex = 0.0;
+      if (lane < 4 * R) {
+        const int r = lane / 4, k = lane % 4;
+        ex = exp((sh->rates[r] * pend_bl) * sh->l[k]);
+      }
+      // P_pend[r][i][j] at lane r*16 + i*4 + j (R*16 <= 64)
+      double pij = 0.0;
+      {
+        const int r = (lane / 16) % R, i = (lane / 4) % 4, j = lane % 4;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const double exk = __shfl(ex, r * 4 + k, 64);
+          pij = pij + (exk * sh->x[i * 4 + k]) * sh->xit[j * 4 + k];
+        }
+      }
+      const int n_t = sh->n_slots * RS;
+      for (int base = 0; base < n_t; base += 64) {
+        const int t = base + lane;
+        const int slot = t / RS, r = (t / 4) % R, i = t % 4;
+        const int c = sh->slot_code[slot < sh->n_slots ? slot : 0];
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const double p = __shfl(pij, r * 16 + i * 4 + j, 64);
+          acc = acc + p * sh->lv[c * 4 + j];
+        }
+        if (t < n_t) sT[slot * RS + r * 4 + i] = (Real)acc;
+      }
+    }
+  }
+
+  // One likelihood probe (Model.log_like3 at the current three-taxon state).
+  __device__ __forceinline__ double log_like() {
+    const bool new_pend = !(pend_bl == c_pend);
+    const bool new_dist = !(dist_bl == c_dist && prox_bl == c_prox);
+    tables(new_pend, new_dist);
+    if (lane == 0) sh->new_dist = new_dist;
+    command(CMD_PROBE);
+    double tot = 0.0;
+    for (int w = 0; w < n_workers; w++) tot += sh->part[w];
+    c_pend = pend_bl;
+    c_dist = dist_bl;
+    c_prox = prox_bl;
+    n_calls++;
+    n_dist_probes += new_dist;
+    const double ll = (tot - (double)M * log_R) + PP_LOG2 * (esum + 0.0);
+    last_ll = ll;
+    have_last = true;
+    return ll;
+  }
+
+  // ---- Minimization.brent with the GSL Brent minimiser ---------------------
+  // kind 0: pendant (three_tax.ml:38-39), 1: distal (three_tax.ml:41-45)
+  __device__ __forceinline__ double fprobe(int kind, double v, int &err) {
+    if (kind == 0) {
+      pend_bl = v;
+    } else {
+      const double prox = (dist_bl + prox_bl) - v;
+      if (!(prox >= 0.)) err = ST_ASSERT;
+      dist_bl = v;
+      prox_bl = prox;
+    }
+    return -log_like();
+  }
+
+  __device__ __forceinline__ int brent(int kind, double raw_start, double left, double right, double tol,
+                       int max_iters, double *result) {
+    int err = 0;
+    if (left >= raw_start || raw_start >= right) return ST_INVALID_START;
+    const double righty = fprobe(kind, right, err);  // (f left, f right) evaluates right first
+    if (err) return err;
+    const double lefty = fprobe(kind, left, err);
+    if (err) return err;
+    const double smaller = lefty < righty ? left : right;
+    const double miny = lefty <= righty ? lefty : righty;  // Stdlib.min
+    double prev = raw_start, pv;
+    for (int iter = 1;; iter++) {
+      pv = fprobe(kind, prev, err);
+      if (err) return err;
+      if (pv < miny) break;
+      if (fabs(prev - smaller) < tol) { *result = smaller; return ST_OK; }  // FoundMin
+      if (iter > max_iters) return ST_FIND_START;
+      prev = (prev + smaller) / 2.;
+    }
+    // gsl_min_fminimizer_set: f(lower), f(upper), f(start) are the values just
+    // computed (the three-taxon state is already at start, as after the set).
+    double x_lower = left, x_upper = right, x_min = prev;
+    double f_lower = lefty, f_upper = righty, f_min = pv;
+    if (!isfinite(f_lower) || !isfinite(f_upper) || !isfinite(f_min)) return ST_GSL;
+    if (x_lower > x_upper || x_min >= x_upper || x_min <= x_lower) return ST_GSL;
+    if (f_min >= f_lower || f_min >= f_upper) return ST_GSL;
+    const double golden = 0.3819660;
+    double v = x_lower + golden * (x_upper - x_lower), w_ = v;
+    double sd = 0., se = 0.;
+    double f_v = fprobe(kind, v, err);
+    if (err) return err;
+    if (!isfinite(f_v)) return ST_GSL;
+    double f_w = f_v;
+    for (int step = 1;; step++) {
+      if (step > max_iters) return ST_MAXITER;
+      const double x_left = x_lower, x_right = x_upper, z = x_min, f_z = f_min;
+      double d = se, e = sd, u;
+      const double w_lower = z - x_left, w_upper = x_right - z;
+      const double tl = 1.4901161193847656e-08 * fabs(z);
+      double p = 0, q = 0, r = 0;
+      const double midpoint = 0.5 * (x_left + x_right);
+      if (fabs(e) > tl) {
+        r = (z - w_) * (f_z - f_v);
+        q = (z - v) * (f_z - f_w);
+        p = (z - v) * q - (z - w_) * r;
+        q = 2 * (q - r);
+        if (q > 0) p = -p; else q = -q;
+        r = e;
+        e = d;
+      }
+      if (fabs(p) < fabs(0.5 * q * r) && p < q * w_lower && p < q * w_upper) {
+        const double t2 = 2 * tl;
+        d = p / q;
+        u = z + d;
+        if ((u - x_left) < t2 || (x_right - u) < t2) d = (z < midpoint) ? tl : -tl;
+      } else {
+        e = (z < midpoint) ? x_right - z : -(z - x_left);
+        d = golden * e;
+      }
+      if (fabs(d) >= tl) u = z + d;
+      else u = z + ((d > 0) ? tl : -tl);
+      se = e;
+      sd = d;
+      const double f_u = fprobe(kind, u, err);
+      if (err) return err;
+      if (!isfinite(f_u)) return ST_GSL;
+      if (f_u <= f_z) {
+        if (u < z) { x_upper = z; f_upper = f_z; }
+        else { x_lower = z; f_lower = f_z; }
+        v = w_; f_v = f_w; w_ = z; f_w = f_z;
+        x_min = u; f_min = f_u;
+      } else {
+        if (u < z) { x_lower = u; f_lower = f_u; }
+        else { x_upper = u; f_upper = f_u; }
+        if (f_u <= f_w || w_ == z) { v = w_; f_v = f_w; w_ = u; f_w = f_u; }
+        else if (f_u <= f_v || v == z || v == w_) { v = u; f_v = f_u; }
+      }
+      if (x_upper - x_lower < tol) { *result = x_min; return ST_OK; }
+    }
+  }
+
+  // three_tax.ml:48-58
+  __device__ __forceinline__ int optimize_something(int kind, double tol, double start_v, double min_v,
+                                    double max_v, double *res) {
+    if (min_v == max_v) { *res = start_v; return ST_OK; }
+    return brent(kind, start_v, min_v, max_v, tol, 100, res);
+  }
+
+  // three_tax.ml:70-83
+  __device__ __forceinline__ int optimize(double tol, double max_pend, int max_iter) {
+    double prev_pend = pend_bl, prev_dist = dist_bl;
+    for (int step = 1;; step++) {
+      if (step > max_iter) return ST_MAXITER;
+      double cp, cd;
+      int st = optimize_something(0, tol, pend_bl, 1e-8, max_pend, &cp);
+      if (st) return st;
+      st = optimize_something(1, tol, dist_bl, 0., dist_bl + prox_bl, &cd);
+      if (st) return st;
+      if (fabs(prev_pend - cp) > tol || fabs(prev_dist - cd) > tol) {
+        prev_pend = cp;
+        prev_dist = cd;
+      } else {
+        return ST_OK;
+      }
+    }
+  }
+
+  // core.ml:225-239 (set_tt_edges; the GLVs are already gathered)
+  __device__ __forceinline__ void set_state(double cut_bl, double pendant, double distal) {
+    pend_bl = pendant;
+    dist_bl = distal;
+    prox_bl = cut_bl - distal;
+    have_last = false;
+  }
+
+  // core.ml:255-281: optimize; on a GSL error retry once from the default
+  // branch lengths; ExceededMaxIter keeps the current state.  The result is
+  // Three_tax.get_results (three_tax.ml:85) at the last probe.
+  __device__ __forceinline__ int safe_optimize(double tol, double max_pend, double start_pend, double cut_bl,
+                               double *ll, double *pend, double *dist) {
+    for (int attempt = 0; attempt < 2; attempt++) {
+      const int st = optimize(tol, max_pend, 200);
+      if (st == ST_OK || st == ST_MAXITER) {
+        *ll = have_last ? last_ll : log_like();
+        *pend = pend_bl;
+        *dist = dist_bl;
+        return st;
+      }
+      if (st != ST_GSL || attempt == 1) return st;
+      set_state(cut_bl, start_pend, cut_bl / 2.);
+    }
+    return ST_OK;
+  }
+};
+
+}  // namespace pp

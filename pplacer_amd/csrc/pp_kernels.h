@@ -15,14 +15,21 @@ struct Res {  // one optimised (query, edge): Three_tax.get_results + location
   int flags;
 };
 
+struct CodeLogArgs {
+  const double *sn;     // [N][Epad][R*4] FP64 supernodes
+  int N, Epad, n_inf;
+  const double *ttab;   // [n_inf][R*4] character LVs evolved by start_pend
+  double *lt64;         // [N][n_inf+1][Epad], or
+  float *lt32;          // the same in FP32 (when non-null)
+};
+
 struct Stage1Args {
-  const void *sn;         // [N][Epad][R*4] Real
-  int N, E, Epad, n_codes, n_inf;
-  const int *sn_e;        // [N][Epad]
-  const uint8_t *tcodes;  // [n_tiles][N][TQ]
+  const void *lt;         // [N][C1][Epad] Real: log util per (site, character, edge)
+  int N, E, Epad, C1;     // C1 = n_inf + 1
+  const int *sn_e;        // [N][Epad] supernode exponents
+  const uint8_t *tcodes;  // [n_tiles][N][TQ], gap = n_inf
   const int *tile_lo, *tile_hi, *tile_q;
   const int *qM;          // masked length per (batch-local) query
-  const double *ttab;     // [n_codes][R*4] query rows evolved by start_pend
   double log_R;
   double *scores;         // [Q][E]
 };

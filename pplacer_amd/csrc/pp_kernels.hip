@@ -2,6 +2,7 @@
 #include <climits>
 
 #include "pp_engine.h"
+#include "pp_engine_mw.h"
 #include "pp_kernels.h"
 
 namespace pp {
@@ -10,94 +11,111 @@ namespace pp {
 // Stage 1: Glv.masked_logdot of every (query, edge) pair (core.ml:187-194,
 // gmix_model.ml:194-198, linear_c.c:465-539).
 //
-// One lane per edge, TQ queries per workgroup.  The lane streams its edge's
-// supernode rows site by site from the site-major array sn[N][Epad][R*4]
-// (64 lanes read 64 consecutive rows: one contiguous 8 KB (FP64) burst) and
-// reuses each row for the TQ queries of the tile, whose characters sit in
-// tcodes[tile][site][TQ] (wave-uniform loads).  Queries are tiled in order
-// of their first informative column, so a tile's columns overlap.
-// The per-site util is the stub's sequential (rate, state) sum; its log is
-// taken through frexp mantissas folded over kLogGroup sites (below).
+// A query's stage-1 GLV at a site is its character's LV evolved by
+// start_pend, so the per-site util of (query, edge) depends only on
+// (site, character, edge).  code_log_kernel evaluates log(util) once for
+// every informative character, in the stub's order (rates outer, states
+// inner, FP64), into lt[N][C1][Epad] (C1 = n_inf + 1; row n_inf holds 0.0 for
+// non-informative characters).  stage1_kernel then sums, per (query, edge),
+// the rows its characters select, site by site in column order: the same
+// sequential FP64 sum of the same logs as ten_masked_logdot_c.  The supernode
+// exponents are summed as integers alongside.
 // ---------------------------------------------------------------------------
-template <typename Real> struct Frexp;
-template <> struct Frexp<double> {
-  static __device__ __forceinline__ double mant(double x) { return __builtin_amdgcn_frexp_mant(x); }
-  static __device__ __forceinline__ int expo(double x) { return __builtin_amdgcn_frexp_exp(x); }
-  static __device__ __forceinline__ double logd(double x) { return log(x); }
-};
-template <> struct Frexp<float> {
-  static __device__ __forceinline__ float mant(float x) { return __builtin_amdgcn_frexp_mantf(x); }
-  static __device__ __forceinline__ int expo(float x) { return __builtin_amdgcn_frexp_expf(x); }
-  static __device__ __forceinline__ double logd(float x) { return (double)__logf(x); }
-};
-
-// Sites whose utils are folded into one log: the product of kLogGroup
-// frexp mantissas stays >= 2^-kLogGroup, and the binary exponents are summed
-// as integers, so log(util_1 ... util_k) = log(prod mantissas) + (sum exps) ln 2
-// has no underflow and needs one log per kLogGroup sites.
-constexpr int kLogGroup = 8;
-
-template <typename Real, int R, int TQ>
-__global__ __launch_bounds__(256) void stage1_kernel(Stage1Args a) {
-  constexpr int RS = R * 4;
-  using Ops = RealOps<Real>;
-  using F = Frexp<Real>;
-  __shared__ Real tt[kMaxCodes * RS];
-  for (int t = threadIdx.x; t < a.n_codes * RS; t += blockDim.x) tt[t] = (Real)a.ttab[t];
+template <int RS>
+__global__ __launch_bounds__(256) void code_log_kernel(CodeLogArgs a) {
+  __shared__ double tt[kMaxCodes * RS];
+  for (int t = threadIdx.x; t < a.n_inf * RS; t += blockDim.x) tt[t] = a.ttab[t];
   __syncthreads();
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // < Epad by grid construction
+  const int site = blockIdx.y;
+  double s[RS];
+  const double *sp = a.sn + ((size_t)site * a.Epad + e) * RS;
+#pragma unroll
+  for (int t = 0; t < RS; t++) s[t] = sp[t];
+  const size_t base = (size_t)site * (a.n_inf + 1) * a.Epad + e;
+  for (int c = 0; c < a.n_inf; c++) {
+    const double *z = tt + c * RS;
+    double util = 0.0;
+#pragma unroll
+    for (int t = 0; t < RS; t++) util = util + z[t] * s[t];
+    const double l = log(util);
+    if (a.lt32) a.lt32[base + (size_t)c * a.Epad] = (float)l;
+    else a.lt64[base + (size_t)c * a.Epad] = l;
+  }
+  if (a.lt32) a.lt32[base + (size_t)a.n_inf * a.Epad] = 0.0f;
+  else a.lt64[base + (size_t)a.n_inf * a.Epad] = 0.0;
+}
+
+// Character rows are read with buffer loads: the lane's byte offset is fixed,
+// the row offset is a scalar, and the resource is rebased once per site.
+template <typename Real> struct RowLoad;
+template <> struct RowLoad<double> {
+  static __device__ __forceinline__ double get(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  }
+};
+template <> struct RowLoad<float> {
+  static __device__ __forceinline__ float get(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  }
+};
+constexpr int kBufferDword3 = 0x00020000;  // gfx9 raw buffer resource word 3
+
+template <typename Real, int TQ>
+__global__ __launch_bounds__(256) void stage1_kernel(Stage1Args a) {
+  static_assert(TQ % 4 == 0, "codes are read four per dword");
   const int e = blockIdx.x * blockDim.x + threadIdx.x;  // < Epad by grid construction
   const int tile = blockIdx.y;
   const int lo = a.tile_lo[tile], hi = a.tile_hi[tile];
-  const int n_inf = a.n_inf;
+  const unsigned gap = (unsigned)(a.C1 - 1);
   double ll[TQ];
-  Real pm[TQ];   // running product of util mantissas
-  int pe[TQ];    // util binary exponents + supernode exponents
-  int cnt[TQ];   // sites folded into pm
+  int pe[TQ];
 #pragma unroll
-  for (int j = 0; j < TQ; j++) { ll[j] = 0.0; pm[j] = 1; pe[j] = 0; cnt[j] = 0; }
-  const Real *snp = reinterpret_cast<const Real *>(a.sn) + ((size_t)lo * a.Epad + e) * RS;
+  for (int j = 0; j < TQ; j++) { ll[j] = 0.0; pe[j] = 0; }
+  const unsigned rowb = (unsigned)a.Epad * sizeof(Real);  // bytes between character rows
+  const size_t siteb = (size_t)a.C1 * rowb;               // bytes per site
+  const char *ltp = reinterpret_cast<const char *>(a.lt) + (size_t)lo * siteb;
+  const unsigned voff = (unsigned)e * sizeof(Real);
   const int *sep = a.sn_e + (size_t)lo * a.Epad + e;
-  const uint8_t *cp = a.tcodes + ((size_t)tile * a.N + lo) * TQ;
-  const size_t sstride = (size_t)a.Epad * RS;
+  const unsigned *cp = reinterpret_cast<const unsigned *>(a.tcodes + ((size_t)tile * a.N + lo) * TQ);
   for (int site = lo; site < hi; ++site) {
-    Real s[RS];
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(ltp), (short)0, (int)siteb, kBufferDword3);
+    unsigned cj[TQ];
 #pragma unroll
-    for (int t = 0; t < RS; t++) s[t] = snp[t];
+    for (int w = 0; w < TQ / 4; w++) {
+      const unsigned word = __builtin_amdgcn_readfirstlane(cp[w]);
+#pragma unroll
+      for (int b = 0; b < 4; b++) cj[4 * w + b] = (word >> (8 * b)) & 0xffu;
+    }
+    Real v[TQ];
+#pragma unroll
+    for (int j = 0; j < TQ; j++) {
+      v[j] = 0;
+      if (cj[j] != gap) v[j] = RowLoad<Real>::get(rs, voff, cj[j] * rowb);
+    }
     const int ex = *sep;
 #pragma unroll
     for (int j = 0; j < TQ; j++) {
-      const int c = cp[j];
-      if (c < n_inf) {
-        const Real *z = tt + c * RS;
-        Real util = 0;
-#pragma unroll
-        for (int t = 0; t < RS; t++) util = Ops::madd(util, z[t], s[t]);
-        pm[j] *= F::mant(util);
-        pe[j] += F::expo(util) + ex;
-        if (++cnt[j] == kLogGroup) {
-          ll[j] += F::logd(pm[j]);
-          pm[j] = 1;
-          cnt[j] = 0;
-        }
-      }
+      ll[j] += (double)v[j];
+      pe[j] += cj[j] != gap ? ex : 0;
     }
-    snp += sstride;
+    ltp += siteb;
     sep += a.Epad;
-    cp += TQ;
+    cp += TQ / 4;
   }
   if (e < a.E) {
 #pragma unroll
     for (int j = 0; j < TQ; j++) {
       const int q = a.tile_q[tile * TQ + j];
-      if (q >= 0) {
-        const double l = ll[j] + F::logd(pm[j]);
-        a.scores[(size_t)q * a.E + e] = (l - (double)a.qM[q] * a.log_R) + PP_LOG2 * (double)pe[j];
-      }
+      if (q >= 0)
+        a.scores[(size_t)q * a.E + e] = (ll[j] - (double)a.qM[q] * a.log_R) + PP_LOG2 * (double)pe[j];
     }
   }
 }
 
-// tcodes[tile][site][j] = codes[tile_q[tile*TQ+j]][site] (gap code for empty slots)
+// tcodes[tile][site][j] = codes[tile_q[tile*TQ+j]][site]; empty slots and
+// non-informative characters read the gap code (= n_inf)
 __global__ void tile_codes_kernel(const uint8_t *codes, const int *tile_q, int n_tiles, int N,
                                   int TQ, uint8_t gap, uint8_t *tcodes) {
   const size_t total = (size_t)n_tiles * N * TQ;
@@ -108,7 +126,8 @@ __global__ void tile_codes_kernel(const uint8_t *codes, const int *tile_q, int n
     const int site = (int)(ts % N);
     const int tile = (int)(ts / N);
     const int q = tile_q[tile * TQ + j];
-    tcodes[i] = q >= 0 ? codes[(size_t)q * N + site] : gap;
+    const uint8_t c = q >= 0 ? codes[(size_t)q * N + site] : gap;
+    tcodes[i] = c < gap ? c : gap;
   }
 }
 
@@ -199,6 +218,188 @@ __global__ __launch_bounds__(MAXBD) void pitch_kernel(PitchArgs a) {
       atomicAdd(a.n_calls + 3, (unsigned long long)(dprobes * M));
     }
   }
+}
+
+
+// ---------------------------------------------------------------------------
+// Master/worker forms of the stage-2 kernels (pp_engine_mw.h): wave 0 runs
+// the control, waves 1.. hold one masked site per lane.
+// ---------------------------------------------------------------------------
+
+// wave-level (key, edge) minimum, ties by smaller edge id; uniform result
+__device__ __forceinline__ void wave_argmin(unsigned long long &key, int &idx) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const unsigned long long k2 = __shfl_xor(key, m, 64);
+    const int i2 = __shfl_xor(idx, m, 64);
+    if (k2 < key || (k2 == key && i2 < idx)) { key = k2; idx = i2; }
+  }
+}
+
+template <typename Real, int R, int MAXBD>
+__global__ __launch_bounds__(MAXBD) void pitch_mw_kernel(PitchArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  using L = MWLayout<Real, R>;
+  MWShared *sh = reinterpret_cast<MWShared *>(smem);
+  mw_copy_model(sh, a.model);
+  __syncthreads();
+  const Real *darr = reinterpret_cast<const Real *>(a.darr);
+  const Real *parr = reinterpret_cast<const Real *>(a.parr);
+  if (wave_id() != 0) {
+    mw_worker<Real, R>(smem, a.N, a.codes, darr, parr, a.darr_e, a.parr_e);
+    return;
+  }
+  MWMaster<Real, R> m;
+  m.init(smem, a.model);
+  // taken[e]: read and written only by the lane that owns edge e (e % 64)
+  uint8_t *taken = reinterpret_cast<uint8_t *>(smem + L::bytes(a.N));
+  const int lane = m.lane;
+  for (;;) {
+    int qi = 0;
+    if (lane == 0) qi = atomicAdd(a.counter, 1);
+    qi = __builtin_amdgcn_readfirstlane(qi);  // wave-uniform: branches below hold barriers
+    if (qi >= a.nq) break;
+    const int q = a.qlist[qi];
+    const int M = m.load_query(a.codes + (size_t)q * a.N, a.N, q);
+    if (M == 0 || M > 64 * m.n_workers) {
+      if (lane == 0) { a.n_res[q] = 0; a.status[q] = M == 0 ? PP_Q_NO_INFORMATIVE : PP_Q_RANK_EXHAUSTED; }
+      continue;
+    }
+    for (int e = lane; e < a.E; e += 64) taken[e] = 0;
+    const double *sc = a.scores + (size_t)q * a.E;
+    double like_record = -INFINITY;
+    long long n_strikes = 0, calls = 0, dprobes = 0;
+    int n_res = 0, status = PP_Q_OK;
+    for (;;) {
+      // next edge of the stable descending sort of the scores (fig.ml:143-147)
+      unsigned long long bk = ~0ull;
+      int bi = INT_MAX;
+      for (int e = lane; e < a.E; e += 64) {
+        if (taken[e]) continue;
+        const unsigned long long k = desc_key(sc[e]);
+        if (k < bk) { bk = k; bi = e; }  // e increases: ties keep the smaller id
+      }
+      wave_argmin(bk, bi);
+      bi = __builtin_amdgcn_readfirstlane(bi);
+      if (bi == INT_MAX) break;
+      if (lane == (bi & 63)) taken[bi] = 1;
+      const int loc = bi;
+      m.gather(loc);
+      const double cut = a.bl[loc];
+      m.set_state(cut, a.start_pend, cut / 2.);
+      m.n_calls = 0;
+      m.n_dist_probes = 0;
+      double ll, pend, dist;
+      const int st = m.safe_optimize(a.initial_tol, a.max_pend, a.start_pend, cut, &ll, &pend, &dist);
+      calls += m.n_calls;
+      dprobes += m.n_dist_probes;
+      if (st == ST_GSL) continue;  // "GSL problem with location": skipped (core.ml:286-297)
+      if (st != ST_OK && st != ST_MAXITER) {
+        status = st == ST_FIND_START ? PP_Q_FIND_START
+                 : st == ST_INVALID_START ? PP_Q_INVALID_START : PP_Q_ASSERT;
+        break;
+      }
+      if (lane == 0) {
+        Res r;
+        r.ll = ll; r.pend = pend; r.dist = dist; r.loc = loc; r.flags = st == ST_MAXITER;
+        a.res[(size_t)q * a.cap + n_res] = r;
+      }
+      const int prev_len = n_res;
+      n_res++;
+      if (prev_len >= a.t_max_pitches) break;
+      else if (ll > like_record) like_record = ll;
+      else if (ll < like_record - a.strike_box) {
+        if (n_strikes + 1 >= a.t_max_strikes) break;
+        n_strikes++;
+      }
+      if (n_res >= a.cap) break;
+    }
+    if (lane == 0) {
+      a.n_res[q] = n_res;
+      a.status[q] = (status == PP_Q_OK && n_res == 0) ? PP_Q_NO_RESULTS : status;
+      atomicAdd(a.n_calls, (unsigned long long)calls);
+      atomicAdd(a.n_pitched, (unsigned long long)n_res);
+      atomicAdd(a.n_calls + 2, (unsigned long long)((calls - dprobes) * M));
+      atomicAdd(a.n_calls + 3, (unsigned long long)(dprobes * M));
+    }
+  }
+  m.command(CMD_EXIT);
+}
+
+template <typename Real, int R, int MAXBD>
+__global__ __launch_bounds__(MAXBD) void optimize_mw_kernel(OptArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  MWShared *sh = reinterpret_cast<MWShared *>(smem);
+  mw_copy_model(sh, a.model);
+  __syncthreads();
+  const Real *darr = reinterpret_cast<const Real *>(a.darr);
+  const Real *parr = reinterpret_cast<const Real *>(a.parr);
+  if (wave_id() != 0) {
+    mw_worker<Real, R>(smem, a.N, a.codes, darr, parr, a.darr_e, a.parr_e);
+    return;
+  }
+  MWMaster<Real, R> m;
+  m.init(smem, a.model);
+  const int lane = m.lane;
+  for (;;) {
+    int job = 0;
+    if (lane == 0) job = atomicAdd(a.counter, 1);
+    job = __builtin_amdgcn_readfirstlane(job);
+    if (job >= a.n_jobs) break;
+    int q, loc;
+    double pend0, dist0;
+    Res *slot = nullptr;
+    int jj = 0;
+    if (a.mode == 1) {
+      const int qi = job / a.K, k = job % a.K;
+      q = a.qlist[qi];
+      if (k >= a.n_keep[q]) continue;
+      slot = a.kept + (size_t)q * a.K + k;
+      loc = slot->loc;
+      pend0 = slot->pend;
+      dist0 = slot->dist;
+    } else {
+      jj = a.qlist ? a.qlist[job] : job;
+      q = a.job_q[jj];
+      loc = a.job_loc[jj];
+      pend0 = a.job_pend[jj];
+      dist0 = a.job_dist[jj];
+    }
+    const int M = m.load_query(a.codes + (size_t)q * a.N, a.N, q);
+    if (M == 0 || M > 64 * m.n_workers) {
+      if (lane == 0 && a.mode == 0) a.out_status[jj] = M == 0 ? PP_Q_NO_INFORMATIVE : PP_Q_RANK_EXHAUSTED;
+      continue;
+    }
+    m.gather(loc);
+    const double cut = a.bl[loc];
+    m.set_state(cut, pend0, dist0);
+    m.n_calls = 0;
+    m.n_dist_probes = 0;
+    double ll, pend, dist;
+    const int st = m.safe_optimize(a.tol, a.max_pend, a.start_pend, cut, &ll, &pend, &dist);
+    if (lane == 0) {
+      atomicAdd(a.n_calls, (unsigned long long)m.n_calls);
+      atomicAdd(a.n_calls + 2, (unsigned long long)((m.n_calls - m.n_dist_probes) * M));
+      atomicAdd(a.n_calls + 3, (unsigned long long)(m.n_dist_probes * M));
+      if (a.mode == 1) {
+        if (st == ST_OK || st == ST_MAXITER) {
+          slot->ll = ll;
+          slot->pend = pend;
+          slot->dist = dist;
+        } else if (st != ST_GSL) {
+          a.status[q] = st == ST_FIND_START ? PP_Q_FIND_START
+                        : st == ST_INVALID_START ? PP_Q_INVALID_START : PP_Q_ASSERT;
+        }  // a GSL problem in the final optimisation keeps the initial result
+      } else {
+        a.out_ll[jj] = ll;
+        a.out_pend[jj] = pend;
+        a.out_dist[jj] = dist;
+        a.out_calls[jj] = (int)m.n_calls;
+        a.out_status[jj] = st;
+      }
+    }
+  }
+  m.command(CMD_EXIT);
 }
 
 // ---------------------------------------------------------------------------
