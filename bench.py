@@ -47,6 +47,22 @@ def kernel_work(stats, R=4, S=4, w=8):
     return s1_ref_flops, s1_gather_bytes, s2_flops
 
 
+def copy_calibration(nbytes):
+    """One device-to-device hipMemcpy of nbytes (the runtime's blit kernel): a
+    kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE readings."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)):
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) or hip.hipDeviceSynchronize():
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
 def cpu_baseline(wl, queries, n_sample, threads):
     """The oracle (FP64 restatement of pplacer, one query per worker like a
     pplacer -j child) on a bounded sample of the same workload."""
@@ -114,15 +130,9 @@ def main():
     pl.upload_queries(queries)
     prefs = placer.default_prefs()
     if args.copy_calibration_mb:
-        import torch
-        n = args.copy_calibration_mb * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        copy_calibration(args.copy_calibration_mb << 20)
         log("copy calibration: %d MiB read + %d MiB written" % (args.copy_calibration_mb,
                                                                args.copy_calibration_mb))
-        del src, dst
 
     def run_form(prec_name):
         prec = PP_FP64 if prec_name == "fp64" else PP_FP32

@@ -45,8 +45,8 @@ namespace {
 
 constexpr int kTQ = 16;           // queries per stage-1 tile
 constexpr int kChunk = 32768;     // queries per device batch
-constexpr int kBuckets = 15;      // masked-length classes of 64 sites (M <= 960)
-constexpr int kMaxM = 64 * kBuckets;
+constexpr int kSPL = 2;           // masked sites per worker lane in stage 2
+constexpr int kBuckets = 15;      // masked-length classes of 64*kSPL sites
 
 struct DBuf {
   void *p = nullptr;
@@ -76,16 +76,13 @@ int ensure(DBuf &b, size_t count, T **out) {
     if (st_) return st_;                     \
   } while (0)
 
-double now_ms() {
-  using namespace std::chrono;
-  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
-}
 
 }  // namespace
 
 struct pp_ctx {
   int device = 0;
   int n_cu = 256;
+  size_t max_lds = 65536;  // bytes of LDS one workgroup may use
   hipStream_t stream = nullptr;
   HostModel model;
   DBuf b_model;
@@ -363,8 +360,8 @@ int gpu_precompute(pp_ctx *c, const TreeDesc &t, const char *leaf_seqs) {
   return PP_EINVAL;
 }
 
-int bucket_of(int M) { return M <= 0 ? 0 : (M - 1) / 64; }  // workers = bucket+1
-int block_of_bucket(int b) { return 64 * (b + 2); }           // + one master wave
+int bucket_of(int M) { return M <= 0 ? 0 : (M - 1) / (64 * kSPL); }  // workers = bucket+1
+int block_of_bucket(int b) { return 64 * (b + 2); }                      // + one master wave
 
 // The stage-1 log-util table for one start_pend (see code_log_kernel).
 // Built from the FP64 supernodes for both forms; the FP32 form stores it in
@@ -456,25 +453,44 @@ int occupancy_grid(const void *kern, int bd, size_t lds, int n_cu, int n_items) 
   return std::max(1, std::min(n_items, nb * n_cu));
 }
 
-// Master/worker stage-2 kernels: one master wave + ceil(M/64) worker waves.
-// Three launch-bound classes give the register allocator the budget of the
+// Stage-2 kernels: one master wave + ceil(M/(64*kSPL)) worker waves.  Three
+// launch-bound classes give the register allocator the budget of the
 // occupancy the block size allows (512 lanes: 2 waves/SIMD, 768: 3, 1024: 4).
+// LDS: header and tables, masked-site list, the pitch kernel's per-edge
+// flags, cached products of the worker lanes' sites.
+template <typename Real, int R>
+size_t stage2_lds(const pp_ctx *c, int bd, int taken_bytes) {
+  using L = MWLayout<Real, R>;
+  return L::align(L::bytes(c->N) + (size_t)taken_bytes) + MWProducts<Real, R * 4>::bytes((bd - 64) * kSPL);
+}
+
+// Longest masked length whose stage-2 workgroup fits in LDS (FP64, the
+// larger form); longer queries get PP_Q_RANK_EXHAUSTED ("query too long").
+int max_masked_sites(const pp_ctx *c) {
+  int m = 0;
+  for (int b = 0; b < kBuckets; b++) {
+    if (stage2_lds<double, 4>(c, block_of_bucket(b), c->E) > c->max_lds) break;
+    m = 64 * kSPL * (b + 1);
+  }
+  return m;
+}
+
 template <typename Real, int R, int MAXBD>
 int launch_pitch_b(pp_ctx *c, PitchArgs a, int bd) {
-  const size_t lds = MWLayout<Real, R>::bytes(c->N) + (size_t)c->E + 64;
-  const void *k = reinterpret_cast<const void *>(&pitch_mw_kernel<Real, R, MAXBD>);
+  const size_t lds = stage2_lds<Real, R>(c, bd, c->E);
+  const void *k = reinterpret_cast<const void *>(&pitch_mw_kernel<Real, R, MAXBD, kSPL>);
   const int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.nq);
-  hipLaunchKernelGGL((pitch_mw_kernel<Real, R, MAXBD>), dim3(grid), dim3(bd), lds, c->stream, a);
+  hipLaunchKernelGGL((pitch_mw_kernel<Real, R, MAXBD, kSPL>), dim3(grid), dim3(bd), lds, c->stream, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
 
 template <typename Real, int R, int MAXBD>
 int launch_opt_b(pp_ctx *c, OptArgs a, int bd) {
-  const size_t lds = MWLayout<Real, R>::bytes(c->N) + 64;
-  const void *k = reinterpret_cast<const void *>(&optimize_mw_kernel<Real, R, MAXBD>);
+  const size_t lds = stage2_lds<Real, R>(c, bd, 0);
+  const void *k = reinterpret_cast<const void *>(&optimize_mw_kernel<Real, R, MAXBD, kSPL>);
   const int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.n_jobs);
-  hipLaunchKernelGGL((optimize_mw_kernel<Real, R, MAXBD>), dim3(grid), dim3(bd), lds, c->stream, a);
+  hipLaunchKernelGGL((optimize_mw_kernel<Real, R, MAXBD, kSPL>), dim3(grid), dim3(bd), lds, c->stream, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
@@ -518,6 +534,7 @@ int queries_upload(pp_ctx *c, int n_queries, const char *queries) {
   std::vector<int> lo(n_queries), hi(n_queries), M(n_queries), st(n_queries, PP_Q_OK);
   std::vector<uint8_t> codes((size_t)n_queries * N);
   const int n_inf = c->model.n_inf;
+  const int max_m = max_masked_sites(c);
   for (int q = 0; q < n_queries; q++) {
     const char *s = queries + (size_t)q * N;
     uint8_t *o = codes.data() + (size_t)q * N;
@@ -529,7 +546,7 @@ int queries_upload(pp_ctx *c, int n_queries, const char *queries) {
       if (code < n_inf) { m++; l = std::min(l, k); h = k + 1; }
     }
     if (st[q] == PP_Q_OK && m == 0) st[q] = PP_Q_NO_INFORMATIVE;
-    if (st[q] == PP_Q_OK && m > kMaxM) st[q] = PP_Q_RANK_EXHAUSTED;
+    if (st[q] == PP_Q_OK && m > max_m) st[q] = PP_Q_RANK_EXHAUSTED;
     lo[q] = m ? l : 0;
     hi[q] = m ? h : 0;
     M[q] = m;
@@ -804,6 +821,9 @@ int pp_ctx_create(int device, pp_ctx **out) {
   hipDeviceProp_t prop;
   HIPCHECK(hipGetDeviceProperties(&prop, device));
   c->n_cu = prop.multiProcessorCount;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+    c->max_lds = (size_t)lds;
   HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   *out = c;
   return PP_OK;

@@ -6,14 +6,17 @@
 // Brent minimiser (minimization.ml:11-100) -- on wave-uniform registers, and
 // builds each probe's transition matrices and evolved pendant rows with its 64
 // lanes (Diagd.multi_exp, diagd.ml:79-100; gemmish_c, linear_c.c:29-90).  Lane
-// j of worker wave w holds masked site 64*(w-1)+j of the query: the distal
-// and proximal GLV rows of the current edge (Glv.mask_into,
-// gmix_model.ml:121-141) and the cached (pi*prox_ev)*dist_ev products of
-// ten_log_like3_c (linear_c.c:367-424).  Each command is one LDS word and two
+// j of worker wave w holds, in registers, the distal and proximal GLV rows of
+// the current edge (Glv.mask_into, gmix_model.ml:121-141) at its masked
+// sites; the cached (pi*prox_ev)*dist_ev products of ten_log_like3_c
+// (linear_c.c:367-424) are kept in LDS.  Each command is one LDS word and two
 // barriers:
 //
 //   master: [tables, command] -> barrier A -> .............................. -> barrier B -> [sum]
 //   worker: ..................   barrier A -> [site work, wave partial sum] -> barrier B
+//
+// With SPL sites per lane, worker lane j of wave w holds masked sites
+// 64*(w-1)+j + k*64*W for k < SPL.
 //
 // Control state and site data live in different waves, so the register
 // allocator sizes the kernel for max(control, data) rather than their sum;
@@ -29,18 +32,22 @@
 #pragma once
 
 #include <climits>
+#include <type_traits>
 
 #include "pp_engine.h"
 
 namespace pp {
 
-enum { CMD_EXIT = 0, CMD_QUERY = 1, CMD_GATHER = 2, CMD_PROBE = 3 };
+// CMD_PROBE_DIST: a probe whose proximal/distal lengths changed (re-evolve)
+enum { CMD_EXIT = 0, CMD_QUERY = 1, CMD_GATHER = 2, CMD_PROBE = 3, CMD_PROBE_DIST = 4 };
 
 struct MWShared {
   double x[16], l[4], xit[16], statd[4], rates[kMaxRates];
   double lv[kMaxCodes * 4];
-  int op, new_dist, n_slots, M, q, loc;
-  int code_slot[kMaxCodes];  // character code -> pendant row of this query
+  int op, n_slots, M, q, loc;
+  // character code -> pendant row of this query; the one-hot characters
+  // (codes 0-3) always use rows 0-3
+  int code_slot[kMaxCodes];
   int slot_code[kMaxCodes];
   double part[16];           // per-worker-wave sums of site logs
   long long ipart[16];       // per-worker-wave sums of site exponents
@@ -58,6 +65,7 @@ struct MWLayout {
   static __device__ Real *Pp(char *s) { return T(s) + kMaxCodes * RS; }
   static __device__ Real *Pd(char *s) { return Pp(s) + R * 16; }
   static __device__ int *sidx(char *s) { return reinterpret_cast<int *>(Pd(s) + R * 16); }
+  static __host__ __device__ size_t align(size_t off) { return (off + 15) & ~size_t(15); }
 };
 
 // Wave-uniform branch on the wave index (wave 0 = master).
@@ -73,84 +81,157 @@ __device__ __forceinline__ void mw_copy_model(MWShared *sh, const DevModel *dm) 
 // ---------------------------------------------------------------------------
 // worker waves
 // ---------------------------------------------------------------------------
-template <typename Real, int R>
-__device__ __forceinline__ void mw_worker(char *smem, int N, const uint8_t *codes, const Real *darr,
-                          const Real *parr, const int *darr_e, const int *parr_e) {
+// Sum over the lane's active sites of log(util).  In FP64 the SPL logs are
+// taken as one: log(prod of frexp mantissas) + (sum of exponents) * ln 2 (the
+// mantissa product of SPL <= 8 sites stays >= 2^-8, so nothing underflows).
+template <typename Real, int SPL>
+__device__ __forceinline__ double site_log_sum(const Real (&util)[SPL], const bool (&active)[SPL]) {
+  if constexpr (std::is_same<Real, double>::value && SPL > 1) {
+    static_assert(SPL <= 8, "mantissa product must stay normal");
+    double m = 1.0;
+    int e = 0;
+#pragma unroll
+    for (int k = 0; k < SPL; k++)
+      if (active[k]) {
+        m *= __builtin_amdgcn_frexp_mant(util[k]);
+        e += __builtin_amdgcn_frexp_exp(util[k]);
+      }
+    return log(m) + (double)e * PP_LOG2;
+  } else {
+    double lsum = 0.0;
+#pragma unroll
+    for (int k = 0; k < SPL; k++)
+      if (active[k]) lsum += RealOps<Real>::logd(util[k]);
+    return lsum;
+  }
+}
+
+// The cached products of masked site i live in LDS in groups of four sites:
+// byte (i/4)*kGroupBytes + t*4*sizeof(Real) + (i%4)*sizeof(Real).  Each lane's
+// sixteen products are immediate offsets from one address, and the 32-byte pad
+// per group spreads a wave's accesses over all banks.
+template <typename Real, int RS>
+struct MWProducts {
+  static constexpr int kGroupBytes = RS * 4 * (int)sizeof(Real) + 32;
+  static __host__ __device__ size_t bytes(int sites) { return (size_t)((sites + 3) / 4) * kGroupBytes; }
+  static __device__ __forceinline__ int base(int i) { return (i >> 2) * kGroupBytes + (i & 3) * (int)sizeof(Real); }
+  static __device__ __forceinline__ Real &at(char *wl, int base, int t) {
+    return *reinterpret_cast<Real *>(wl + base + t * 4 * (int)sizeof(Real));
+  }
+};
+
+template <typename Real, int R, int SPL>
+__device__ __forceinline__ void mw_worker(char *smem, char *wl, int N, const uint8_t *codes,
+                                          const Real *darr, const Real *parr, const int *darr_e,
+                                          const int *parr_e) {
   using L = MWLayout<Real, R>;
   using Ops = RealOps<Real>;
   constexpr int RS = R * 4;
+  using W = MWProducts<Real, RS>;
   MWShared *sh = reinterpret_cast<MWShared *>(smem);
   const Real *sT = L::T(smem), *sPp = L::Pp(smem), *sPd = L::Pd(smem);
   const int *sidx = L::sidx(smem);
   const int lane = threadIdx.x & 63;
   const int wv = wave_id() - 1;
+  const int stride = (int)blockDim.x - 64;  // worker lanes
   const int my = wv * 64 + lane;
-  Real dO[RS], pO[RS], w[RS];
-  int site = 0, slot = 0;
-  bool active = false;
-  for (;;) {
+  int site[SPL], slot[SPL], wb[SPL];
+  bool active[SPL];
+#pragma unroll
+  for (int k = 0; k < SPL; k++) wb[k] = W::base(my + k * stride);
+  // The command stream is QUERY (GATHER PROBE*)* ... EXIT; the nested loops
+  // follow it, so each edge's rows are defined once per GATHER and stay in
+  // place through that edge's probes.
+  __syncthreads();  // A
+  int op = sh->op;
+  while (op == CMD_QUERY) {
+#pragma unroll
+    for (int k = 0; k < SPL; k++) {
+      const int i = my + k * stride;
+      active[k] = i < sh->M;
+      site[k] = active[k] ? sidx[i] : 0;
+      slot[k] = active[k] ? sh->code_slot[codes[(size_t)sh->q * N + site[k]]] : 0;
+    }
+    __syncthreads();  // B
     __syncthreads();  // A
-    const int op = sh->op;
-    if (op == CMD_EXIT) break;
-    if (op == CMD_QUERY) {
-      active = my < sh->M;
-      if (active) {
-        site = sidx[my];
-        slot = sh->code_slot[codes[(size_t)sh->q * N + site]];
-      }
-    } else if (op == CMD_GATHER) {
+    op = sh->op;
+    while (op == CMD_GATHER) {
+      // Inactive slots load row 0 (any valid row).
+      Real dO[SPL][RS], pO[SPL][RS];
       long long es = 0;
-      if (active) {
-        const size_t row = (size_t)sh->loc * N + site;
+#pragma unroll
+      for (int k = 0; k < SPL; k++) {
+        const size_t row = active[k] ? (size_t)sh->loc * N + site[k] : 0;
         const Real *d = darr + row * RS;
         const Real *p = parr + row * RS;
 #pragma unroll
-        for (int t = 0; t < RS; t++) { dO[t] = d[t]; pO[t] = p[t]; }
-        es = (long long)darr_e[row] + (long long)parr_e[row];
+        for (int t = 0; t < RS; t++) { dO[k][t] = d[t]; pO[k][t] = p[t]; }
+        if (active[k]) es += (long long)darr_e[row] + (long long)parr_e[row];
       }
       es = wave_sum_ll(es);
       if (lane == 0) sh->ipart[wv] = es;
-    } else {  // CMD_PROBE
-      double lsum = 0.0;
-      if (active) {
-        if (sh->new_dist) {
+      __syncthreads();  // B
+      __syncthreads();  // A
+      op = sh->op;
+      while (op == CMD_PROBE || op == CMD_PROBE_DIST) {
+        // Sums start from their first product: 0 + p == p up to the sign of
+        // a zero, which no later operation can observe.
+        Real util[SPL];
+        if (op == CMD_PROBE_DIST) {
+          // util accumulates as each product is formed, in (rate, state) order
 #pragma unroll
           for (int r = 0; r < R; r++)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-              Real xv = 0, yv = 0;
+              const Real *pp = sPp + r * 16 + i * 4, *pd = sPd + r * 16 + i * 4;
+              const Real pi = (Real)sh->statd[i];
+              const int t = r * 4 + i;
 #pragma unroll
-              for (int j = 0; j < 4; j++) {
-                xv = Ops::madd(xv, sPp[r * 16 + i * 4 + j], pO[r * 4 + j]);
-                yv = Ops::madd(yv, sPd[r * 16 + i * 4 + j], dO[r * 4 + j]);
+              for (int k = 0; k < SPL; k++) {
+                Real xv = pp[0] * pO[k][r * 4], yv = pd[0] * dO[k][r * 4];
+#pragma unroll
+                for (int j = 1; j < 4; j++) {
+                  xv = Ops::madd(xv, pp[j], pO[k][r * 4 + j]);
+                  yv = Ops::madd(yv, pd[j], dO[k][r * 4 + j]);
+                }
+                const Real wv_ = (pi * xv) * yv;
+                W::at(wl, wb[k], t) = wv_;
+                const Real z = sT[slot[k] * RS + t];
+                util[k] = t == 0 ? wv_ * z : Ops::madd(util[k], wv_, z);
               }
-              w[r * 4 + i] = ((Real)sh->statd[i] * xv) * yv;
+            }
+        } else {
+#pragma unroll
+          for (int t = 0; t < RS; t++)
+#pragma unroll
+            for (int k = 0; k < SPL; k++) {
+              const Real wv_ = W::at(wl, wb[k], t), z = sT[slot[k] * RS + t];
+              util[k] = t == 0 ? wv_ * z : Ops::madd(util[k], wv_, z);
             }
         }
-        const Real *z = sT + slot * RS;
-        Real util = 0;
-#pragma unroll
-        for (int t = 0; t < RS; t++) util = Ops::madd(util, w[t], z[t]);
-        lsum = Ops::logd(util);
+        double lsum = site_log_sum<Real, SPL>(util, active);
+        lsum = wave_sum(lsum);
+        if (lane == 0) sh->part[wv] = lsum;
+        __syncthreads();  // B
+        __syncthreads();  // A
+        op = sh->op;
       }
-      lsum = wave_sum(lsum);
-      if (lane == 0) sh->part[wv] = lsum;
     }
-    __syncthreads();  // B
   }
+  // op == CMD_EXIT
 }
 
 // ---------------------------------------------------------------------------
 // master wave
 // ---------------------------------------------------------------------------
-template <typename Real, int R>
+template <typename Real, int R, int SPL>
 struct MWMaster {
   using L = MWLayout<Real, R>;
   static constexpr int RS = R * 4;
   MWShared *sh;
   Real *sT, *sPp, *sPd;
   int *sidx;
-  int lane, n_workers, n_codes, n_inf;
+  int lane, n_workers, cap_sites, n_codes, n_inf;
   double log_R;
 
   // Three_tax state (last probe) and caches
@@ -169,6 +250,7 @@ struct MWMaster {
     sidx = L::sidx(smem);
     lane = threadIdx.x & 63;
     n_workers = (blockDim.x >> 6) - 1;
+    cap_sites = 64 * SPL * n_workers;
     n_codes = dm->n_codes;
     n_inf = dm->n_inf;
     log_R = dm->log_R;
@@ -207,8 +289,9 @@ struct MWMaster {
     for (int k = lo; k < hi; k++)
       if (codes[k] < n_inf) sidx[pos++] = k;
     if (lane == 0) {
-      int ns = 0;
+      int ns = 4;
       for (int c = 0; c < n_codes; c++) {
+        if (c < 4) { sh->code_slot[c] = c; sh->slot_code[c] = c; continue; }
         sh->code_slot[c] = (present >> c) & 1u ? ns : 0;
         if ((present >> c) & 1u) sh->slot_code[ns++] = c;
       }
@@ -218,7 +301,7 @@ struct MWMaster {
     }
     M = total;
     c_pend = __builtin_nan("");  // the pendant rows are per query
-    if (total > 0 && total <= 64 * n_workers) command(CMD_QUERY);
+    if (total > 0 && total <= cap_sites) command(CMD_QUERY);
     return total;
   }
 
@@ -226,8 +309,12 @@ struct MWMaster {
   __device__ __forceinline__ void gather(int loc) {
     if (lane == 0) sh->loc = loc;
     command(CMD_GATHER);
+    long long iv[16];
+#pragma unroll
+    for (int w = 0; w < 16; w++) iv[w] = w < n_workers ? sh->ipart[w] : 0;
     long long es = 0;
-    for (int w = 0; w < n_workers; w++) es += sh->ipart[w];
+#pragma unroll
+    for (int w = 0; w < 16; w++) es += iv[w];
     esum = (double)es;
     c_dist = c_prox = __builtin_nan("");
     have_last = false;
@@ -273,11 +360,18 @@ struct MWMaster {
           pij = pij + (exk * sh->x[i * 4 + k]) * sh->xit[j * 4 + k];
         }
       }
-      const int n_t = sh->n_slots * RS;
+      // One-hot characters: their evolved rows are columns of P (every other
+      // product in gemmish_c is an exact zero), row j entry (r, i) = P[r][i][j].
+      if (lane < R * 16) {
+        const int r = lane / 16, i = (lane / 4) % 4, j = lane % 4;
+        sT[j * RS + r * 4 + i] = (Real)pij;
+      }
+      // Ambiguity codes present in the query: rows 4.. by the stub's sum.
+      const int n_t = (sh->n_slots - 4) * RS;
       for (int base = 0; base < n_t; base += 64) {
         const int t = base + lane;
-        const int slot = t / RS, r = (t / 4) % R, i = t % 4;
-        const int c = sh->slot_code[slot < sh->n_slots ? slot : 0];
+        const int slot = 4 + t / RS, r = (t / 4) % R, i = t % 4;
+        const int c = sh->slot_code[t < n_t ? slot : 0];
         double acc = 0.0;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -294,10 +388,14 @@ struct MWMaster {
     const bool new_pend = !(pend_bl == c_pend);
     const bool new_dist = !(dist_bl == c_dist && prox_bl == c_prox);
     tables(new_pend, new_dist);
-    if (lane == 0) sh->new_dist = new_dist;
-    command(CMD_PROBE);
-    double tot = 0.0;
-    for (int w = 0; w < n_workers; w++) tot += sh->part[w];
+    command(new_dist ? CMD_PROBE_DIST : CMD_PROBE);
+    // partials of waves >= n_workers read as 0.0; the sum is in wave order
+    double pv[16];
+#pragma unroll
+    for (int w = 0; w < 16; w++) pv[w] = w < n_workers ? sh->part[w] : 0.0;
+    double tot = pv[0];
+#pragma unroll
+    for (int w = 1; w < 16; w++) tot += pv[w];
     c_pend = pend_bl;
     c_dist = dist_bl;
     c_prox = prox_bl;

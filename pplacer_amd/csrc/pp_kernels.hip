@@ -223,7 +223,7 @@ __global__ __launch_bounds__(MAXBD) void pitch_kernel(PitchArgs a) {
 
 // ---------------------------------------------------------------------------
 // Master/worker forms of the stage-2 kernels (pp_engine_mw.h): wave 0 runs
-// the control, waves 1.. hold one masked site per lane.
+// the control, waves 1.. hold SPL masked sites per lane.
 // ---------------------------------------------------------------------------
 
 // wave-level (key, edge) minimum, ties by smaller edge id; uniform result
@@ -236,7 +236,7 @@ __device__ __forceinline__ void wave_argmin(unsigned long long &key, int &idx) {
   }
 }
 
-template <typename Real, int R, int MAXBD>
+template <typename Real, int R, int MAXBD, int SPL>
 __global__ __launch_bounds__(MAXBD) void pitch_mw_kernel(PitchArgs a) {
   extern __shared__ __align__(16) char smem[];
   using L = MWLayout<Real, R>;
@@ -245,14 +245,15 @@ __global__ __launch_bounds__(MAXBD) void pitch_mw_kernel(PitchArgs a) {
   __syncthreads();
   const Real *darr = reinterpret_cast<const Real *>(a.darr);
   const Real *parr = reinterpret_cast<const Real *>(a.parr);
-  if (wave_id() != 0) {
-    mw_worker<Real, R>(smem, a.N, a.codes, darr, parr, a.darr_e, a.parr_e);
-    return;
-  }
-  MWMaster<Real, R> m;
-  m.init(smem, a.model);
   // taken[e]: read and written only by the lane that owns edge e (e % 64)
   uint8_t *taken = reinterpret_cast<uint8_t *>(smem + L::bytes(a.N));
+  if (wave_id() != 0) {
+    char *wl = smem + L::align(L::bytes(a.N) + a.E);
+    mw_worker<Real, R, SPL>(smem, wl, a.N, a.codes, darr, parr, a.darr_e, a.parr_e);
+    return;
+  }
+  MWMaster<Real, R, SPL> m;
+  m.init(smem, a.model);
   const int lane = m.lane;
   for (;;) {
     int qi = 0;
@@ -261,7 +262,7 @@ __global__ __launch_bounds__(MAXBD) void pitch_mw_kernel(PitchArgs a) {
     if (qi >= a.nq) break;
     const int q = a.qlist[qi];
     const int M = m.load_query(a.codes + (size_t)q * a.N, a.N, q);
-    if (M == 0 || M > 64 * m.n_workers) {
+    if (M == 0 || M > m.cap_sites) {
       if (lane == 0) { a.n_res[q] = 0; a.status[q] = M == 0 ? PP_Q_NO_INFORMATIVE : PP_Q_RANK_EXHAUSTED; }
       continue;
     }
@@ -326,8 +327,9 @@ __global__ __launch_bounds__(MAXBD) void pitch_mw_kernel(PitchArgs a) {
   m.command(CMD_EXIT);
 }
 
-template <typename Real, int R, int MAXBD>
+template <typename Real, int R, int MAXBD, int SPL>
 __global__ __launch_bounds__(MAXBD) void optimize_mw_kernel(OptArgs a) {
+  using L = MWLayout<Real, R>;
   extern __shared__ __align__(16) char smem[];
   MWShared *sh = reinterpret_cast<MWShared *>(smem);
   mw_copy_model(sh, a.model);
@@ -335,10 +337,11 @@ __global__ __launch_bounds__(MAXBD) void optimize_mw_kernel(OptArgs a) {
   const Real *darr = reinterpret_cast<const Real *>(a.darr);
   const Real *parr = reinterpret_cast<const Real *>(a.parr);
   if (wave_id() != 0) {
-    mw_worker<Real, R>(smem, a.N, a.codes, darr, parr, a.darr_e, a.parr_e);
+    char *wl = smem + L::align(L::bytes(a.N));
+    mw_worker<Real, R, SPL>(smem, wl, a.N, a.codes, darr, parr, a.darr_e, a.parr_e);
     return;
   }
-  MWMaster<Real, R> m;
+  MWMaster<Real, R, SPL> m;
   m.init(smem, a.model);
   const int lane = m.lane;
   for (;;) {
@@ -366,7 +369,7 @@ __global__ __launch_bounds__(MAXBD) void optimize_mw_kernel(OptArgs a) {
       dist0 = a.job_dist[jj];
     }
     const int M = m.load_query(a.codes + (size_t)q * a.N, a.N, q);
-    if (M == 0 || M > 64 * m.n_workers) {
+    if (M == 0 || M > m.cap_sites) {
       if (lane == 0 && a.mode == 0) a.out_status[jj] = M == 0 ? PP_Q_NO_INFORMATIVE : PP_Q_RANK_EXHAUSTED;
       continue;
     }
