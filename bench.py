@@ -32,6 +32,28 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def pitch_flops(stats, R=4, S=4):
+    """Algorithmic flops of the baseball kernel, priced with the reference's
+    per-probe work (U2): a pendant probe evolves the pendant GLV (2*R*S*S per
+    masked site) and runs log_like3 (4*R*S); a distal probe evolves the
+    proximal and distal GLVs (2 * 2*R*S*S) and runs log_like3."""
+    return ((2 * R * S * S + 4 * R * S) * stats.n_site_probes_pend_pitch
+            + (4 * R * S * S + 4 * R * S) * stats.n_site_probes_dist_pitch)
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 2 and not line.startswith("#") and f[0] == kernel:
+            return float(f[1])
+    return None
+
+
 def kernel_work(stats, R=4, S=4, w=8):
     """Algorithmic work of the stage-1 and stage-2 kernels.
     Stage 1 per (query, edge, masked site): the reference's masked_logdot
@@ -169,6 +191,9 @@ def main():
             "stage1_ref_equiv_tflops": s1_ref_flops / (st.t_stage1_ms * 1e-3) / 1e12,
             "stage1_gather_GBps": s1_gather_bytes / (st.t_stage1_ms * 1e-3) / 1e9,
             "stage2_tflops": s2_flops / ((st.t_pitch_ms + st.t_refine_ms) * 1e-3) / 1e12,
+            "pitch_u2_tflops": pitch_flops(st) / (st.t_pitch_ms * 1e-3) / 1e12,
+            "pitch_launches": st.n_pitch_launches,
+            "pitch_avg_launch_ms": st.t_pitch_ms / max(1, st.n_pitch_launches),
             "n_logdots": st.n_logdots, "n_like_calls": st.n_like_calls,
             "n_pitched": st.n_pitched, "n_launches": st.n_launches,
         }
@@ -178,17 +203,27 @@ def main():
     kern = max([("stage1_kernel", st_main["t_stage1_ms"]), ("pitch_kernel", st_main["t_pitch_ms"]),
                 ("refine_kernels", st_main["t_refine_ms"])], key=lambda kv: kv[1])[0]
     peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
+    real = "double" if args.precision == "fp64" else "float"
     if kern == "stage1_kernel":
-        roofline = {"bound": "hbm", "kernel": kern,
+        name = "stage1_kernel<%s,16>" % real
+        roofline = {"bound": "hbm", "kernel": name,
                     "achieved": st_main["stage1_gather_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": st_main["stage1_gather_GBps"] / HBM_PEAK_GBS, "traffic": None,
+                    "frac": st_main["stage1_gather_GBps"] / HBM_PEAK_GBS,
+                    "traffic": read_traffic(name),
                     "note": "gathered table bytes; most are served by L1/L2"}
     else:
-        achieved = st_main["stage2_tflops"]
+        name = "pitch_mw_kernel<%s,4,512,2>" % real
+        achieved = st_main["pitch_u2_tflops"]
         roofline = {"bound": "mfma",
                     "pipe": "vector ALU %s (4-state contractions do not use MFMA)" % args.precision,
-                    "kernel": kern, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                    "frac": achieved / peak, "traffic": None}
+                    "kernel": name, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                    "frac": achieved / peak, "traffic": read_traffic(name),
+                    "avg_launch_ms": st_main["pitch_avg_launch_ms"],
+                    "launches": st_main["pitch_launches"],
+                    "work": "reference U2 flops of the pitched probes: (2RS^2+4RS) per masked site "
+                            "of a pendant probe, (4RS^2+4RS) of a distal probe"}
+        if kern != "pitch_kernel":
+            roofline["note"] = "the refinement kernels took longer than the baseball kernel"
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -180,6 +180,9 @@ typedef struct {
   int64_t n_site_probes_dist; /* sum over probes that re-evolve prox/dist of M */
   int64_t n_launches;         /* kernel launches issued */
   double t_table_ms;          /* stage-1 log-util table build (part of the ranking time) */
+  int64_t n_site_probes_pend_pitch; /* n_site_probes_pend of the baseball kernels alone */
+  int64_t n_site_probes_dist_pitch; /* n_site_probes_dist of the baseball kernels alone */
+  int64_t n_pitch_launches;         /* launches of the baseball kernel */
 } pp_stats;
 
 /* Stage 1 for a batch: scores[q][e] = Glv.masked_logdot of the query (full

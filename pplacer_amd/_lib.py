@@ -41,7 +41,9 @@ class Stats(C.Structure):
                 ("t_stage1_ms", C.c_double), ("t_pitch_ms", C.c_double),
                 ("t_refine_ms", C.c_double), ("n_site_pairs", C.c_int64),
                 ("n_site_probes_pend", C.c_int64), ("n_site_probes_dist", C.c_int64),
-                ("n_launches", C.c_int64), ("t_table_ms", C.c_double)]
+                ("n_launches", C.c_int64), ("t_table_ms", C.c_double),
+                ("n_site_probes_pend_pitch", C.c_int64), ("n_site_probes_dist_pitch", C.c_int64),
+                ("n_pitch_launches", C.c_int64)]
 
 
 _dp = C.POINTER(C.c_double)

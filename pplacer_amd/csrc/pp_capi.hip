@@ -605,12 +605,12 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   ENSURE(c->b_nkeep, chunk, &d_nkeep);
   ENSURE(c->b_qlist, chunk, &d_qlist);
   ENSURE(c->b_counter, 64, &d_counter);
-  ENSURE(c->b_stats, 4, &d_stats);
-  HIPCHECK(hipMemsetAsync(d_stats, 0, 4 * sizeof(unsigned long long), c->stream));
+  ENSURE(c->b_stats, 6, &d_stats);
+  HIPCHECK(hipMemsetAsync(d_stats, 0, 6 * sizeof(unsigned long long), c->stream));
   HIPCHECK(hipMemcpyAsync(d_status, c->qstat.data(), (size_t)Q * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemsetAsync(d_nout, 0, (size_t)std::max(Q, 1) * 4, c->stream));
   double t_s1 = 0, t_pitch = 0, t_ref = 0;
-  long long launches = 0;
+  long long launches = 0, pitch_launches = 0;
   hipEvent_t ev0, ev1, ev1b, ev2, ev3;
   HIPCHECK(hipEventCreate(&ev0));
   HIPCHECK(hipEventCreate(&ev1));
@@ -682,6 +682,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
       st = fp32 ? dispatch_pitch<float>(c, p, block_of_bucket(b)) : dispatch_pitch<double>(c, p, block_of_bucket(b));
       if (st) return st;
       launches++;
+      pitch_launches++;
     }
     HIPCHECK(hipEventRecord(ev2, c->stream));
     KeepArgs ka;
@@ -748,7 +749,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   }
   for (hipEvent_t e : {ev0, ev1, ev1b, ev2, ev3}) (void)hipEventDestroy(e);
   if (stats) {
-    unsigned long long hs[4];
+    unsigned long long hs[6];
     HIPCHECK(hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost));
     stats->t_stage1_ms = t_s1;
     stats->t_table_ms = t_table;
@@ -767,6 +768,9 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     stats->n_pitched = (int64_t)hs[1];
     stats->n_site_probes_pend = (int64_t)hs[2];
     stats->n_site_probes_dist = (int64_t)hs[3];
+    stats->n_site_probes_pend_pitch = (int64_t)hs[4];
+    stats->n_site_probes_dist_pitch = (int64_t)hs[5];
+    stats->n_pitch_launches = pitch_launches;
     stats->n_launches = launches;
   }
   return PP_OK;

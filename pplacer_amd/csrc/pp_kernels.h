@@ -50,6 +50,8 @@ struct PitchArgs {
   int cap;
   Res *res;  // [Q][cap]
   int *n_res, *status;
+  // n_calls[0]: probes; [2], [3]: sum of M over pendant-only / distal probes
+  // (all stage-2 kernels); [4], [5]: the same for this kernel alone
   unsigned long long *n_calls, *n_pitched;
 };
 
@@ -82,7 +84,7 @@ struct OptArgs {
   const double *job_pend, *job_dist;
   double *out_ll, *out_pend, *out_dist;
   int *out_calls, *out_status;
-  unsigned long long *n_calls;
+  unsigned long long *n_calls;  // [0]: probes; [2], [3]: as in PitchArgs
 };
 
 struct FinalArgs {

@@ -132,98 +132,19 @@ __global__ void tile_codes_kernel(const uint8_t *codes, const int *tile_q, int n
 }
 
 // ---------------------------------------------------------------------------
-// Stage 2a: baseball over the stage-1 ranking, one workgroup per query from
-// a work queue (core.ml:283-320).  The next pitch is the best remaining edge
-// of the stable descending sort of the scores (fig.ml:143-147), selected on
-// demand; each pitch is Three_tax.optimize from the default branch lengths
+// Stage 2 (pp_engine_mw.h): wave 0 of a workgroup runs the control, waves
+// 1.. hold SPL masked sites per lane.
+//
+// pitch_mw_kernel: baseball over the stage-1 ranking, one workgroup per query
+// from a work queue (core.ml:283-320).  The next pitch is the best remaining
+// edge of the stable descending sort of the scores (fig.ml:143-147), selected
+// on demand; each pitch is Three_tax.optimize from the default branch lengths
 // at initial_tolerance.
-// ---------------------------------------------------------------------------
-template <typename Real, int R, int MAXBD>
-__global__ __launch_bounds__(MAXBD) void pitch_kernel(PitchArgs a) {
-  extern __shared__ __align__(16) char smem[];
-  __shared__ int s_job;
-  Engine4<Real, R> eng;
-  eng.init(smem, a.N, a.model);
-  unsigned *taken = reinterpret_cast<unsigned *>(smem + Engine4<Real, R>::engine_lds_bytes(a.N));
-  const Real *darr = reinterpret_cast<const Real *>(a.darr);
-  const Real *parr = reinterpret_cast<const Real *>(a.parr);
-  const int bd = blockDim.x, tid = threadIdx.x;
-  const int words = (a.E + 31) >> 5;
-  for (;;) {
-    if (tid == 0) s_job = atomicAdd(a.counter, 1);
-    __syncthreads();
-    const int qi = s_job;
-    __syncthreads();
-    if (qi >= a.nq) break;
-    const int q = a.qlist[qi];
-    const int M = eng.load_query(a.codes + (size_t)q * a.N, a.N);
-    if (M == 0 || M > bd) {
-      if (tid == 0) { a.n_res[q] = 0; a.status[q] = M == 0 ? PP_Q_NO_INFORMATIVE : PP_Q_RANK_EXHAUSTED; }
-      continue;
-    }
-    for (int t = tid; t < words; t += bd) taken[t] = 0u;
-    __syncthreads();
-    const double *sc = a.scores + (size_t)q * a.E;
-    double like_record = -INFINITY;
-    long long n_strikes = 0, calls = 0, dprobes = 0;
-    int n_res = 0, status = PP_Q_OK;
-    for (;;) {
-      unsigned long long bk = ~0ull;
-      int bi = INT_MAX;
-      for (int e = tid; e < a.E; e += bd) {
-        if (taken[e >> 5] & (1u << (e & 31))) continue;
-        const unsigned long long k = desc_key(sc[e]);
-        if (k < bk || (k == bk && e < bi)) { bk = k; bi = e; }
-      }
-      block_argmin(bk, bi, eng.sh->skey, eng.sh->sidx);
-      if (bi == INT_MAX) break;
-      if (tid == 0) taken[bi >> 5] |= 1u << (bi & 31);
-      const int loc = bi;
-      eng.gather(darr, parr, a.darr_e, a.parr_e, a.N, loc);
-      const double cut = a.bl[loc];
-      eng.set_state(cut, a.start_pend, cut / 2.);
-      eng.n_calls = 0;
-      eng.n_dist_probes = 0;
-      double ll, pend, dist;
-      const int st = eng.safe_optimize(a.initial_tol, a.max_pend, a.start_pend, cut, &ll, &pend, &dist);
-      calls += eng.n_calls;
-      dprobes += eng.n_dist_probes;
-      if (st == ST_GSL) continue;  // "GSL problem with location": skipped (core.ml:286-297)
-      if (st != ST_OK && st != ST_MAXITER) {
-        status = st == ST_FIND_START ? PP_Q_FIND_START
-                 : st == ST_INVALID_START ? PP_Q_INVALID_START : PP_Q_ASSERT;
-        break;
-      }
-      if (tid == 0) {
-        Res r;
-        r.ll = ll; r.pend = pend; r.dist = dist; r.loc = loc; r.flags = st == ST_MAXITER;
-        a.res[(size_t)q * a.cap + n_res] = r;
-      }
-      const int prev_len = n_res;
-      n_res++;
-      if (prev_len >= a.t_max_pitches) break;
-      else if (ll > like_record) like_record = ll;
-      else if (ll < like_record - a.strike_box) {
-        if (n_strikes + 1 >= a.t_max_strikes) break;
-        n_strikes++;
-      }
-      if (n_res >= a.cap) break;
-    }
-    if (tid == 0) {
-      a.n_res[q] = n_res;
-      a.status[q] = (status == PP_Q_OK && n_res == 0) ? PP_Q_NO_RESULTS : status;
-      atomicAdd(a.n_calls, (unsigned long long)calls);
-      atomicAdd(a.n_pitched, (unsigned long long)n_res);
-      atomicAdd(a.n_calls + 2, (unsigned long long)((calls - dprobes) * M));
-      atomicAdd(a.n_calls + 3, (unsigned long long)(dprobes * M));
-    }
-  }
-}
-
-
-// ---------------------------------------------------------------------------
-// Master/worker forms of the stage-2 kernels (pp_engine_mw.h): wave 0 runs
-// the control, waves 1.. hold SPL masked sites per lane.
+//
+// optimize_mw_kernel: Three_tax.optimize jobs, one workgroup per (query,
+// edge, start) from a work queue.  mode 0: explicit job arrays
+// (pp_optimize_batch); mode 1: the final refinement of every kept placement
+// at final_tolerance (core.ml:353-369), whose results replace the kept entries.
 // ---------------------------------------------------------------------------
 
 // wave-level (key, edge) minimum, ties by smaller edge id; uniform result
@@ -322,6 +243,8 @@ __global__ __launch_bounds__(MAXBD) void pitch_mw_kernel(PitchArgs a) {
       atomicAdd(a.n_pitched, (unsigned long long)n_res);
       atomicAdd(a.n_calls + 2, (unsigned long long)((calls - dprobes) * M));
       atomicAdd(a.n_calls + 3, (unsigned long long)(dprobes * M));
+      atomicAdd(a.n_calls + 4, (unsigned long long)((calls - dprobes) * M));
+      atomicAdd(a.n_calls + 5, (unsigned long long)(dprobes * M));
     }
   }
   m.command(CMD_EXIT);
@@ -444,85 +367,6 @@ __global__ void keep_kernel(KeepArgs a) {
     kept[nk++] = k;
   }
   a.n_keep[q] = nk;
-}
-
-// ---------------------------------------------------------------------------
-// Three_tax.optimize jobs: one workgroup per (query, edge, start) from a work
-// queue.  mode 0: explicit job arrays (pp_optimize_batch); mode 1: the
-// final refinement of every kept placement at final_tolerance
-// (core.ml:353-369), whose results replace the kept entries.
-// ---------------------------------------------------------------------------
-template <typename Real, int R, int MAXBD>
-__global__ __launch_bounds__(MAXBD) void optimize_kernel(OptArgs a) {
-  extern __shared__ __align__(16) char smem[];
-  __shared__ int s_job;
-  Engine4<Real, R> eng;
-  eng.init(smem, a.N, a.model);
-  const Real *darr = reinterpret_cast<const Real *>(a.darr);
-  const Real *parr = reinterpret_cast<const Real *>(a.parr);
-  const int tid = threadIdx.x;
-  for (;;) {
-    if (tid == 0) s_job = atomicAdd(a.counter, 1);
-    __syncthreads();
-    const int job = s_job;
-    __syncthreads();
-    if (job >= a.n_jobs) break;
-    int q, loc;
-    double pend0, dist0;
-    Res *slot = nullptr;
-    if (a.mode == 1) {
-      const int qi = job / a.K, k = job % a.K;
-      q = a.qlist[qi];
-      if (k >= a.n_keep[q]) continue;
-      slot = a.kept + (size_t)q * a.K + k;
-      loc = slot->loc;
-      pend0 = slot->pend;
-      dist0 = slot->dist;
-    } else {
-      const int jj = a.qlist ? a.qlist[job] : job;
-      q = a.job_q[jj];
-      loc = a.job_loc[jj];
-      pend0 = a.job_pend[jj];
-      dist0 = a.job_dist[jj];
-    }
-    const int M = eng.load_query(a.codes + (size_t)q * a.N, a.N);
-    if (M == 0 || M > (int)blockDim.x) {
-      if (tid == 0 && a.mode == 0) {
-        const int jj = a.qlist ? a.qlist[job] : job;
-        a.out_status[jj] = M == 0 ? PP_Q_NO_INFORMATIVE : PP_Q_RANK_EXHAUSTED;
-      }
-      continue;
-    }
-    eng.gather(darr, parr, a.darr_e, a.parr_e, a.N, loc);
-    const double cut = a.bl[loc];
-    eng.set_state(cut, pend0, dist0);
-    eng.n_calls = 0;
-    eng.n_dist_probes = 0;
-    double ll, pend, dist;
-    const int st = eng.safe_optimize(a.tol, a.max_pend, a.start_pend, cut, &ll, &pend, &dist);
-    if (tid == 0) {
-      atomicAdd(a.n_calls, (unsigned long long)eng.n_calls);
-      atomicAdd(a.n_calls + 2, (unsigned long long)((eng.n_calls - eng.n_dist_probes) * M));
-      atomicAdd(a.n_calls + 3, (unsigned long long)(eng.n_dist_probes * M));
-      if (a.mode == 1) {
-        if (st == ST_OK || st == ST_MAXITER) {
-          slot->ll = ll;
-          slot->pend = pend;
-          slot->dist = dist;
-        } else if (st != ST_GSL) {
-          a.status[q] = st == ST_FIND_START ? PP_Q_FIND_START
-                        : st == ST_INVALID_START ? PP_Q_INVALID_START : PP_Q_ASSERT;
-        }  // GSL problem in the final optimisation keeps the initial result
-      } else {
-        const int jj = a.qlist ? a.qlist[job] : job;
-        a.out_ll[jj] = ll;
-        a.out_pend[jj] = pend;
-        a.out_dist[jj] = dist;
-        a.out_calls[jj] = (int)eng.n_calls;
-        a.out_status[jj] = st;
-      }
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
