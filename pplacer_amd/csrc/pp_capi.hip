@@ -94,6 +94,7 @@ struct pp_ctx {
   DBuf b_bl, b_d64, b_p64, b_d32, b_p32, b_de, b_pe, b_s64, b_se;
   DBuf b_de32, b_pe32;  // exponents of the FP32 copies
   DBuf b_lt;            // stage-1 log-util table [N][n_inf+1][Epad]
+  DBuf b_scratch;       // 20-state stage-2 pair scratch
   std::vector<double> h_ttab;
   // resident queries (sorted by first informative column)
   int Q = 0;
@@ -155,7 +156,7 @@ int make_fp32_copies(pp_ctx *c);
 
 int upload_glvs(pp_ctx *c, const TreeGlvs &g) {
   const int G = c->G, N = c->N, R = c->model.R, S = c->model.S, Epad = c->Epad;
-  if (S != 4) { set_error("the GPU placement path supports nucleotide models (n_states = 4)"); return PP_EINVAL; }
+  if (S != 4 && S != 20) { set_error("the GPU placement path supports n_states 4 and 20"); return PP_EINVAL; }
   const size_t glv = (size_t)G * N * R * S, snn = (size_t)N * Epad * R * S;
   double *d64, *p64, *s64;
   int *de, *pe, *se;
@@ -285,8 +286,8 @@ int gpu_precompute_t(pp_ctx *c, const TreeDesc &t, const std::vector<uint8_t> &c
     if (!n) continue;
     a.level_nodes = lvl + pos;
     a.n_level = n;
-    const long work = (long)n * N;
-    hipLaunchKernelGGL((pre::distal_level_kernel<RS, S>), dim3((work + 255) / 256), dim3(256), 0, c->stream, a);
+    const long work = (long)n * N * R;
+    hipLaunchKernelGGL((pre::distal_level_kernel<R, S>), dim3((work + 255) / 256), dim3(256), 0, c->stream, a);
     HIPCHECK(hipGetLastError());
     pos += n;
   }
@@ -308,15 +309,15 @@ int gpu_precompute_t(pp_ctx *c, const TreeDesc &t, const std::vector<uint8_t> &c
     if (!n) continue;
     a.level_nodes = dl + pos;
     a.n_level = n;
-    const long work = (long)n * N;
-    hipLaunchKernelGGL((pre::proximal_level_kernel<RS, S>), dim3((work + 255) / 256), dim3(256), 0, c->stream, a);
+    const long work = (long)n * N * R;
+    hipLaunchKernelGGL((pre::proximal_level_kernel<R, S>), dim3((work + 255) / 256), dim3(256), 0, c->stream, a);
     HIPCHECK(hipGetLastError());
     pos += n;
   }
   HIPCHECK(hipMemsetAsync(s64, 0, snn * 8, c->stream));
   HIPCHECK(hipMemsetAsync(se, 0, (size_t)N * Epad * 4, c->stream));
-  const long work = (long)G * N;
-  hipLaunchKernelGGL((pre::finish_kernel<RS, S>), dim3((work + 255) / 256), dim3(256), 0, c->stream, a, statd, E,
+  const long work = (long)G * N * R;
+  hipLaunchKernelGGL((pre::finish_kernel<R, S>), dim3((work + 255) / 256), dim3(256), 0, c->stream, a, statd, E,
                      Epad, s64, se);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -353,9 +354,11 @@ int gpu_precompute(pp_ctx *c, const TreeDesc &t, const char *leaf_seqs) {
       codes[(size_t)id * N + k] = (uint8_t)code;
     }
   }
-  if (S != 4) { set_error("the GPU placement path supports nucleotide models (n_states = 4)"); return PP_EINVAL; }
-  if (R == 4) return gpu_precompute_t<4, 4>(c, t, codes, parent, height, depth);
-  if (R == 1) return gpu_precompute_t<1, 4>(c, t, codes, parent, height, depth);
+  if (S == 4 && R == 4) return gpu_precompute_t<4, 4>(c, t, codes, parent, height, depth);
+  if (S == 4 && R == 1) return gpu_precompute_t<1, 4>(c, t, codes, parent, height, depth);
+  if (S == 20 && R == 4) return gpu_precompute_t<4, 20>(c, t, codes, parent, height, depth);
+  if (S == 20 && R == 1) return gpu_precompute_t<1, 20>(c, t, codes, parent, height, depth);
+  if (S != 4 && S != 20) { set_error("the GPU placement path supports n_states 4 and 20"); return PP_EINVAL; }
   set_error("n_rates must be 1 or 4 on the GPU path");
   return PP_EINVAL;
 }
@@ -385,8 +388,11 @@ int build_code_logs(pp_ctx *c, double start_pend) {
   a.lt64 = std::is_same<Real, double>::value ? reinterpret_cast<double *>(lt) : nullptr;
   a.lt32 = std::is_same<Real, float>::value ? reinterpret_cast<float *>(lt) : nullptr;
   const dim3 grid(Epad / 256, N);
-  if (R == 4) hipLaunchKernelGGL(code_log_kernel<16>, grid, dim3(256), 0, c->stream, a);
-  else if (R == 1) hipLaunchKernelGGL(code_log_kernel<4>, grid, dim3(256), 0, c->stream, a);
+  const int RS = R * c->model.S;
+  if (RS == 16) hipLaunchKernelGGL(code_log_kernel<16>, grid, dim3(256), 0, c->stream, a);
+  else if (RS == 4) hipLaunchKernelGGL(code_log_kernel<4>, grid, dim3(256), 0, c->stream, a);
+  else if (RS == 80) hipLaunchKernelGGL(code_log_kernel<80>, grid, dim3(256), 0, c->stream, a);
+  else if (RS == 20) hipLaunchKernelGGL(code_log_kernel<20>, grid, dim3(256), 0, c->stream, a);
   else { set_error("n_rates must be 1 or 4 on the GPU path"); return PP_EINVAL; }
   HIPCHECK(hipGetLastError());
   return PP_OK;
@@ -464,9 +470,17 @@ size_t stage2_lds(const pp_ctx *c, int bd, int taken_bytes) {
   return L::align(L::bytes(c->N) + (size_t)taken_bytes) + MWProducts<Real, R * 4>::bytes((bd - 64) * kSPL);
 }
 
+// Whether the FP32 form runs stage 2 in FP32.  The 20-state engine evaluates
+// evolves in the eigenbasis of Q, which rebuilds small GLV entries from
+// eigen coordinates of larger magnitude; in FP32 that cancellation loses
+// them, so 20-state stage 2 always runs in FP64 (the FP32 form keeps its
+// FP32 stage-1 table).
+bool stage2_fp32_ok(const pp_ctx *c) { return c->model.S == 4; }
+
 // Longest masked length whose stage-2 workgroup fits in LDS (FP64, the
 // larger form); longer queries get PP_Q_RANK_EXHAUSTED ("query too long").
 int max_masked_sites(const pp_ctx *c) {
+  if (c->model.S == 20) return c->N;  // pair state lives in global scratch
   int m = 0;
   for (int b = 0; b < kBuckets; b++) {
     if (stage2_lds<double, 4>(c, block_of_bucket(b), c->E) > c->max_lds) break;
@@ -509,8 +523,62 @@ int launch_opt(pp_ctx *c, OptArgs a, int bd) {
   return launch_opt_b<Real, R, 1024>(c, a, bd);
 }
 
+// 20-state stage 2: one master + 7 worker waves per query, pairs per lane as
+// the longest masked length needs, per-workgroup pair scratch.
+// Sized for the kernel's full-occupancy grid, so that launches with fewer
+// workgroups never reallocate it while earlier launches may still run.
+template <typename Real>
+int prot_scratch(pp_ctx *c, const void *kern, size_t lds, int cap_pairs, void **out) {
+  const int grid_full = occupancy_grid<Real, 4>(kern, 512, lds, c->n_cu, INT_MAX);
+  Real *p;
+  ENSURE(c->b_scratch, ProtLayout::scratch_reals(cap_pairs) * (size_t)grid_full, &p);
+  *out = p;
+  return PP_OK;
+}
+
+int prot_cap_pairs(const pp_ctx *c) {
+  constexpr int worker_lanes = 512 - 64;
+  int max_m = 1;
+  for (int q = 0; q < c->Q; q++)
+    if (c->qstat[q] == PP_Q_OK) max_m = std::max(max_m, c->qM[q]);
+  const int pairs = max_m * c->model.R;
+  return (pairs + worker_lanes - 1) / worker_lanes * worker_lanes;
+}
+
+template <typename Real, int R>
+int launch_pitch_prot(pp_ctx *c, PitchArgs a) {
+  const size_t lds = ProtLayout::bytes(c->N) + (size_t)c->E;
+  const void *k = reinterpret_cast<const void *>(&pitch_prot_kernel<Real, R>);
+  const int grid = occupancy_grid<Real, R>(k, 512, lds, c->n_cu, a.nq);
+  a.cap_pairs = prot_cap_pairs(c);
+  int st = prot_scratch<Real>(c, k, lds, a.cap_pairs, &a.scratch);
+  if (st) return st;
+  hipLaunchKernelGGL((pitch_prot_kernel<Real, R>), dim3(grid), dim3(512), lds, c->stream, a);
+  HIPCHECK(hipGetLastError());
+  return PP_OK;
+}
+
+template <typename Real, int R>
+int launch_opt_prot(pp_ctx *c, OptArgs a) {
+  const size_t lds = ProtLayout::bytes(c->N);
+  const void *k = reinterpret_cast<const void *>(&optimize_prot_kernel<Real, R>);
+  const int grid = occupancy_grid<Real, R>(k, 512, lds, c->n_cu, a.n_jobs);
+  a.cap_pairs = prot_cap_pairs(c);
+  int st = prot_scratch<Real>(c, k, lds, a.cap_pairs, &a.scratch);
+  if (st) return st;
+  hipLaunchKernelGGL((optimize_prot_kernel<Real, R>), dim3(grid), dim3(512), lds, c->stream, a);
+  HIPCHECK(hipGetLastError());
+  return PP_OK;
+}
+
 template <typename Real>
 int dispatch_pitch(pp_ctx *c, PitchArgs a, int bd) {
+  if (c->model.S == 20) {
+    if (c->model.R == 4) return launch_pitch_prot<Real, 4>(c, a);
+    if (c->model.R == 1) return launch_pitch_prot<Real, 1>(c, a);
+    set_error("n_rates must be 1 or 4 on the GPU path");
+    return PP_EINVAL;
+  }
   if (c->model.R == 4) return launch_pitch<Real, 4>(c, a, bd);
   if (c->model.R == 1) return launch_pitch<Real, 1>(c, a, bd);
   set_error("n_rates must be 1 or 4 on the GPU path");
@@ -518,6 +586,12 @@ int dispatch_pitch(pp_ctx *c, PitchArgs a, int bd) {
 }
 template <typename Real>
 int dispatch_opt(pp_ctx *c, OptArgs a, int bd) {
+  if (c->model.S == 20) {
+    if (c->model.R == 4) return launch_opt_prot<Real, 4>(c, a);
+    if (c->model.R == 1) return launch_opt_prot<Real, 1>(c, a);
+    set_error("n_rates must be 1 or 4 on the GPU path");
+    return PP_EINVAL;
+  }
   if (c->model.R == 4) return launch_opt<Real, 4>(c, a, bd);
   if (c->model.R == 1) return launch_opt<Real, 1>(c, a, bd);
   set_error("n_rates must be 1 or 4 on the GPU path");
@@ -618,6 +692,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   HIPCHECK(hipEventCreate(&ev2));
   HIPCHECK(hipEventCreate(&ev3));
   const bool fp32 = precision == PP_FP32;
+  const bool s2_fp32 = fp32 && stage2_fp32_ok(c);
   float t_table = 0;
   HIPCHECK(hipEventRecord(ev0, c->stream));
   st = fp32 ? build_code_logs<float>(c, prefs->start_pend) : build_code_logs<double>(c, prefs->start_pend);
@@ -653,17 +728,17 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     for (int b = 0; b < kBuckets; b++) {
       const int n = boff[b + 1] - boff[b];
       if (!n) continue;
-      PitchArgs p;
+      PitchArgs p{};
       p.model = c->d_model;
       p.codes = codes;
       p.scores = d_scores;
       p.qlist = d_qlist + boff[b];
       p.nq = n;
       p.counter = d_counter + b;
-      p.darr = fp32 ? c->b_d32.p : c->b_d64.p;
-      p.parr = fp32 ? c->b_p32.p : c->b_p64.p;
-      p.darr_e = reinterpret_cast<const int *>(fp32 ? c->b_de32.p : c->b_de.p);
-      p.parr_e = reinterpret_cast<const int *>(fp32 ? c->b_pe32.p : c->b_pe.p);
+      p.darr = s2_fp32 ? c->b_d32.p : c->b_d64.p;
+      p.parr = s2_fp32 ? c->b_p32.p : c->b_p64.p;
+      p.darr_e = reinterpret_cast<const int *>(s2_fp32 ? c->b_de32.p : c->b_de.p);
+      p.parr_e = reinterpret_cast<const int *>(s2_fp32 ? c->b_pe32.p : c->b_pe.p);
       p.bl = reinterpret_cast<const double *>(c->b_bl.p);
       p.N = N;
       p.E = E;
@@ -679,7 +754,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
       p.status = status;
       p.n_calls = d_stats;
       p.n_pitched = d_stats + 1;
-      st = fp32 ? dispatch_pitch<float>(c, p, block_of_bucket(b)) : dispatch_pitch<double>(c, p, block_of_bucket(b));
+      st = s2_fp32 ? dispatch_pitch<float>(c, p, block_of_bucket(b)) : dispatch_pitch<double>(c, p, block_of_bucket(b));
       if (st) return st;
       launches++;
       pitch_launches++;
@@ -700,7 +775,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     for (int b = 0; b < kBuckets; b++) {
       const int n = boff[b + 1] - boff[b];
       if (!n) continue;
-      OptArgs o;
+      OptArgs o{};
       std::memset(&o, 0, sizeof(o));
       o.model = c->d_model;
       o.codes = codes;
@@ -838,7 +913,7 @@ int pp_ctx_destroy(pp_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   DBuf *bufs[] = {&c->b_model, &c->b_bl, &c->b_d64, &c->b_p64, &c->b_d32, &c->b_p32, &c->b_de,
-                  &c->b_pe, &c->b_s64, &c->b_se, &c->b_de32, &c->b_pe32, &c->b_lt, &c->b_codes, &c->b_qM, &c->b_tcodes,
+                  &c->b_pe, &c->b_s64, &c->b_se, &c->b_de32, &c->b_pe32, &c->b_lt, &c->b_scratch, &c->b_codes, &c->b_qM, &c->b_tcodes,
                   &c->b_tile_lo, &c->b_tile_hi, &c->b_tile_q, &c->b_ttab, &c->b_scores, &c->b_res,
                   &c->b_nres, &c->b_status, &c->b_kept, &c->b_nkeep, &c->b_out, &c->b_nout,
                   &c->b_qlist, &c->b_counter, &c->b_stats};
@@ -1082,7 +1157,7 @@ int pp_optimize_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
   for (int b = 0; b < kBuckets; b++) {
     const int n = boff[b + 1] - boff[b];
     if (!n) continue;
-    OptArgs o;
+    OptArgs o{};
     std::memset(&o, 0, sizeof(o));
     o.model = c->d_model;
     o.codes = reinterpret_cast<const uint8_t *>(c->b_codes.p);
@@ -1092,7 +1167,7 @@ int pp_optimize_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
     o.K = 1;
     o.qlist = d_list + boff[b];
     o.counter = d_counter + b;
-    const bool fp32 = precision == PP_FP32;
+    const bool fp32 = precision == PP_FP32 && stage2_fp32_ok(c);
     o.darr = fp32 ? c->b_d32.p : c->b_d64.p;
     o.parr = fp32 ? c->b_p32.p : c->b_p64.p;
     o.darr_e = reinterpret_cast<const int *>(fp32 ? c->b_de32.p : c->b_de.p);

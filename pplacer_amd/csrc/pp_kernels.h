@@ -53,6 +53,9 @@ struct PitchArgs {
   // n_calls[0]: probes; [2], [3]: sum of M over pendant-only / distal probes
   // (all stage-2 kernels); [4], [5]: the same for this kernel alone
   unsigned long long *n_calls, *n_pitched;
+  // 20-state engine: per-workgroup pair scratch (ProtLayout::scratch_reals(cap_pairs) each)
+  void *scratch;
+  int cap_pairs;
 };
 
 struct KeepArgs {
@@ -85,6 +88,8 @@ struct OptArgs {
   double *out_ll, *out_pend, *out_dist;
   int *out_calls, *out_status;
   unsigned long long *n_calls;  // [0]: probes; [2], [3]: as in PitchArgs
+  void *scratch;                // 20-state engine, as in PitchArgs
+  int cap_pairs;
 };
 
 struct FinalArgs {

@@ -3,6 +3,7 @@
 
 #include "pp_engine.h"
 #include "pp_engine_mw.h"
+#include "pp_engine_prot.h"
 #include "pp_kernels.h"
 
 namespace pp {
@@ -157,24 +158,9 @@ __device__ __forceinline__ void wave_argmin(unsigned long long &key, int &idx) {
   }
 }
 
-template <typename Real, int R, int MAXBD, int SPL>
-__global__ __launch_bounds__(MAXBD) void pitch_mw_kernel(PitchArgs a) {
-  extern __shared__ __align__(16) char smem[];
-  using L = MWLayout<Real, R>;
-  MWShared *sh = reinterpret_cast<MWShared *>(smem);
-  mw_copy_model(sh, a.model);
-  __syncthreads();
-  const Real *darr = reinterpret_cast<const Real *>(a.darr);
-  const Real *parr = reinterpret_cast<const Real *>(a.parr);
-  // taken[e]: read and written only by the lane that owns edge e (e % 64)
-  uint8_t *taken = reinterpret_cast<uint8_t *>(smem + L::bytes(a.N));
-  if (wave_id() != 0) {
-    char *wl = smem + L::align(L::bytes(a.N) + a.E);
-    mw_worker<Real, R, SPL>(smem, wl, a.N, a.codes, darr, parr, a.darr_e, a.parr_e);
-    return;
-  }
-  MWMaster<Real, R, SPL> m;
-  m.init(smem, a.model);
+// Baseball over one work queue of queries, for either engine's master.
+template <class Master>
+__device__ __forceinline__ void pitch_loop(const PitchArgs &a, Master &m, uint8_t *taken) {
   const int lane = m.lane;
   for (;;) {
     int qi = 0;
@@ -251,21 +237,46 @@ __global__ __launch_bounds__(MAXBD) void pitch_mw_kernel(PitchArgs a) {
 }
 
 template <typename Real, int R, int MAXBD, int SPL>
-__global__ __launch_bounds__(MAXBD) void optimize_mw_kernel(OptArgs a) {
-  using L = MWLayout<Real, R>;
+__global__ __launch_bounds__(MAXBD) void pitch_mw_kernel(PitchArgs a) {
   extern __shared__ __align__(16) char smem[];
+  using L = MWLayout<Real, R>;
   MWShared *sh = reinterpret_cast<MWShared *>(smem);
   mw_copy_model(sh, a.model);
   __syncthreads();
   const Real *darr = reinterpret_cast<const Real *>(a.darr);
   const Real *parr = reinterpret_cast<const Real *>(a.parr);
+  // taken[e]: read and written only by the lane that owns edge e (e % 64)
+  uint8_t *taken = reinterpret_cast<uint8_t *>(smem + L::bytes(a.N));
   if (wave_id() != 0) {
-    char *wl = smem + L::align(L::bytes(a.N));
+    char *wl = smem + L::align(L::bytes(a.N) + a.E);
     mw_worker<Real, R, SPL>(smem, wl, a.N, a.codes, darr, parr, a.darr_e, a.parr_e);
     return;
   }
   MWMaster<Real, R, SPL> m;
   m.init(smem, a.model);
+  pitch_loop(a, m, taken);
+}
+
+template <typename Real, int R>
+__global__ __launch_bounds__(512) void pitch_prot_kernel(PitchArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  prot_copy_model(reinterpret_cast<ProtShared *>(smem), a.model);
+  __syncthreads();
+  uint8_t *taken = reinterpret_cast<uint8_t *>(smem + ProtLayout::bytes(a.N));
+  if (wave_id() != 0) {
+    Real *scr = reinterpret_cast<Real *>(a.scratch) + ProtLayout::scratch_reals(a.cap_pairs) * blockIdx.x;
+    prot_worker<Real, R>(smem, scr, a.cap_pairs, a.N, a.codes, reinterpret_cast<const Real *>(a.darr),
+                         reinterpret_cast<const Real *>(a.parr), a.darr_e, a.parr_e);
+    return;
+  }
+  ProtMaster<Real, R> m;
+  m.init(smem, a.model, a.cap_pairs);
+  pitch_loop(a, m, taken);
+}
+
+// Three_tax.optimize jobs of one work queue, for either engine's master.
+template <class Master>
+__device__ __forceinline__ void optimize_loop(const OptArgs &a, Master &m) {
   const int lane = m.lane;
   for (;;) {
     int job = 0;
@@ -326,6 +337,41 @@ __global__ __launch_bounds__(MAXBD) void optimize_mw_kernel(OptArgs a) {
     }
   }
   m.command(CMD_EXIT);
+}
+
+template <typename Real, int R, int MAXBD, int SPL>
+__global__ __launch_bounds__(MAXBD) void optimize_mw_kernel(OptArgs a) {
+  using L = MWLayout<Real, R>;
+  extern __shared__ __align__(16) char smem[];
+  MWShared *sh = reinterpret_cast<MWShared *>(smem);
+  mw_copy_model(sh, a.model);
+  __syncthreads();
+  const Real *darr = reinterpret_cast<const Real *>(a.darr);
+  const Real *parr = reinterpret_cast<const Real *>(a.parr);
+  if (wave_id() != 0) {
+    char *wl = smem + L::align(L::bytes(a.N));
+    mw_worker<Real, R, SPL>(smem, wl, a.N, a.codes, darr, parr, a.darr_e, a.parr_e);
+    return;
+  }
+  MWMaster<Real, R, SPL> m;
+  m.init(smem, a.model);
+  optimize_loop(a, m);
+}
+
+template <typename Real, int R>
+__global__ __launch_bounds__(512) void optimize_prot_kernel(OptArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  prot_copy_model(reinterpret_cast<ProtShared *>(smem), a.model);
+  __syncthreads();
+  if (wave_id() != 0) {
+    Real *scr = reinterpret_cast<Real *>(a.scratch) + ProtLayout::scratch_reals(a.cap_pairs) * blockIdx.x;
+    prot_worker<Real, R>(smem, scr, a.cap_pairs, a.N, a.codes, reinterpret_cast<const Real *>(a.darr),
+                         reinterpret_cast<const Real *>(a.parr), a.darr_e, a.parr_e);
+    return;
+  }
+  ProtMaster<Real, R> m;
+  m.init(smem, a.model, a.cap_pairs);
+  optimize_loop(a, m);
 }
 
 // ---------------------------------------------------------------------------
@@ -510,40 +556,40 @@ __global__ void logsum_kernel(const double *util, const uint16_t *mask, int firs
 // ---------------------------------------------------------------------------
 namespace pre {
 
-template <int RS, int S>
+// One lane per (row, rate): a GLV row [R][S] is held by R consecutive lanes,
+// S values each.  R divides 64, so a row never straddles a wave, and every
+// launch covers whole rows.
+
+// Glv.perhaps_pull_exponent (gmix_model.ml:163-187) on one row: the frexp
+// exponent of each rate's largest state, maximised over the row's R lanes.
+template <int R, int S>
 __device__ __forceinline__ void pull(double *v, int &e, int min_twoexp) {
-  constexpr int R = RS / S;
-  int mx_te = INT_MIN;
+  double mx = v[0];
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    double mx = v[r * S];
+  for (int k = 1; k < S; k++)
+    if (v[k] > mx) mx = v[k];
+  int te;
+  (void)frexp(mx, &te);
 #pragma unroll
-    for (int k = 1; k < S; k++)
-      if (v[r * S + k] > mx) mx = v[r * S + k];
-    int te;
-    (void)frexp(mx, &te);
-    if (te > mx_te) mx_te = te;
-  }
-  if (mx_te < min_twoexp) {
-    const double f = ldexp(0.5, -mx_te + 1);
+  for (int o = 1; o < R; o <<= 1) te = max(te, __shfl_xor(te, o, 64));
+  if (te < min_twoexp) {
+    const double f = ldexp(0.5, -te + 1);
 #pragma unroll
-    for (int t = 0; t < RS; t++) v[t] *= f;
-    e += mx_te;
+    for (int k = 0; k < S; k++) v[k] *= f;
+    e += te;
   }
 }
 
-template <int RS, int S>
-__device__ __forceinline__ void evolve_row(const double *P, const double *src, double *dst) {
-  constexpr int R = RS / S;
+// dst = P_r * src for one rate (gemmish_c order: sequential sum over j)
+template <int S>
+__device__ __forceinline__ void evolve_rate(const double *P, const double *src, double *dst) {
+#pragma unroll 4
+  for (int i = 0; i < S; i++) {
+    double acc = 0;
 #pragma unroll
-  for (int r = 0; r < R; r++)
-#pragma unroll
-    for (int i = 0; i < S; i++) {
-      double acc = 0;
-#pragma unroll
-      for (int j = 0; j < S; j++) acc = acc + P[(r * S + i) * S + j] * src[r * S + j];
-      dst[r * S + i] = acc;
-    }
+    for (int j = 0; j < S; j++) acc = acc + P[i * S + j] * src[j];
+    dst[i] = acc;
+  }
 }
 
 struct TreeArgs {
@@ -560,73 +606,77 @@ struct TreeArgs {
 
 // post-order: distal = product of the children's evolved distals (or the
 // leaf's LV), pulled at 2^-50, then evolv_dist = P(bl) * distal
-template <int RS, int S>
+template <int R, int S>
 __global__ void distal_level_kernel(TreeArgs a) {
+  constexpr int RS = R * S;
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= (long)a.n_level * a.N) return;
-  const int site = (int)(t % a.N);
-  const int node = a.level_nodes[t / a.N];
+  if (t >= (long)a.n_level * a.N * R) return;
+  const int r = (int)(t % R);
+  const long ts = t / R;
+  const int site = (int)(ts % a.N);
+  const int node = a.level_nodes[ts / a.N];
   const size_t row = (size_t)node * a.N + site;
-  double v[RS];
+  double v[S];
   int e = 0;
   const int c0 = a.child_start[node], c1 = a.child_start[node + 1];
   if (c1 == c0) {
     const int code = a.leaf_codes[row];
 #pragma unroll
-    for (int r = 0; r < RS / S; r++)
-#pragma unroll
-      for (int k = 0; k < S; k++) v[r * S + k] = a.lv[code * S + k];
+    for (int k = 0; k < S; k++) v[k] = a.lv[code * S + k];
   } else {
     const size_t r0 = (size_t)a.children[c0] * a.N + site;
 #pragma unroll
-    for (int k = 0; k < RS; k++) v[k] = a.evd[r0 * RS + k];
+    for (int k = 0; k < S; k++) v[k] = a.evd[r0 * RS + r * S + k];
     e = a.evd_e[r0];
     for (int c = c0 + 1; c < c1; c++) {
       const size_t rc = (size_t)a.children[c] * a.N + site;
 #pragma unroll
-      for (int k = 0; k < RS; k++) v[k] = v[k] * a.evd[rc * RS + k];
+      for (int k = 0; k < S; k++) v[k] = v[k] * a.evd[rc * RS + r * S + k];
       e += a.evd_e[rc];
     }
-    pull<RS, S>(v, e, -50);
+    pull<R, S>(v, e, -50);
   }
 #pragma unroll
-  for (int k = 0; k < RS; k++) a.darr[row * RS + k] = v[k];
-  a.darr_e[row] = e;
-  double w[RS];
-  evolve_row<RS, S>(a.P + (size_t)node * RS * S, v, w);
+  for (int k = 0; k < S; k++) a.darr[row * RS + r * S + k] = v[k];
+  if (r == 0) a.darr_e[row] = e;
+  double w[S];
+  evolve_rate<S>(a.P + ((size_t)node * R + r) * S * S, v, w);
 #pragma unroll
-  for (int k = 0; k < RS; k++) a.evd[row * RS + k] = w[k];
-  a.evd_e[row] = e;
+  for (int k = 0; k < S; k++) a.evd[row * RS + r * S + k] = w[k];
+  if (r == 0) a.evd_e[row] = e;
 }
 
 // pre-order: proximal of a child = P(bl_parent) * proximal(parent) times the
 // evolved distals of its siblings in order, pulled at 2^-50
-template <int RS, int S>
+template <int R, int S>
 __global__ void proximal_level_kernel(TreeArgs a) {
+  constexpr int RS = R * S;
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= (long)a.n_level * a.N) return;
-  const int site = (int)(t % a.N);
-  const int child = a.level_nodes[t / a.N];
+  if (t >= (long)a.n_level * a.N * R) return;
+  const int r = (int)(t % R);
+  const long ts = t / R;
+  const int site = (int)(ts % a.N);
+  const int child = a.level_nodes[ts / a.N];
   const int par = a.parent[child];
   const size_t prow = (size_t)par * a.N + site;
-  double src[RS], v[RS];
+  double src[S], v[S];
 #pragma unroll
-  for (int k = 0; k < RS; k++) src[k] = a.parr[prow * RS + k];
+  for (int k = 0; k < S; k++) src[k] = a.parr[prow * RS + r * S + k];
   int e = a.parr_e[prow];
-  evolve_row<RS, S>(a.P + (size_t)par * RS * S, src, v);
+  evolve_rate<S>(a.P + ((size_t)par * R + r) * S * S, src, v);
   for (int c = a.child_start[par]; c < a.child_start[par + 1]; c++) {
     const int sib = a.children[c];
     if (sib == child) continue;
     const size_t rs = (size_t)sib * a.N + site;
 #pragma unroll
-    for (int k = 0; k < RS; k++) v[k] = v[k] * a.evd[rs * RS + k];
+    for (int k = 0; k < S; k++) v[k] = v[k] * a.evd[rs * RS + r * S + k];
     e += a.evd_e[rs];
   }
-  pull<RS, S>(v, e, -50);
+  pull<R, S>(v, e, -50);
   const size_t row = (size_t)child * a.N + site;
 #pragma unroll
-  for (int k = 0; k < RS; k++) a.parr[row * RS + k] = v[k];
-  a.parr_e[row] = e;
+  for (int k = 0; k < S; k++) a.parr[row * RS + r * S + k] = v[k];
+  if (r == 0) a.parr_e[row] = e;
 }
 
 __global__ void fill_unit_kernel(double *row, int *e, int N, int RS) {
@@ -638,37 +688,42 @@ __global__ void fill_unit_kernel(double *row, int *e, int N, int RS) {
 
 // pull both arrays at 2^-10 (pplacer_run.ml:331-336), then the supernodes
 // pi * P(bl/2) darr * P(bl/2) parr (glv_arr.ml:46-56), written site-major
-// for stage 1, with FP32 copies of everything
-template <int RS, int S>
+// for stage 1
+template <int R, int S>
 __global__ void finish_kernel(TreeArgs a, const double *statd, int E, int Epad, double *sn64,
                               int *sn_e) {
+  constexpr int RS = R * S;
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= (long)a.n_nodes * a.N) return;
-  const int site = (int)(t % a.N);
-  const int node = (int)(t / a.N);
+  if (t >= (long)a.n_nodes * a.N * R) return;
+  const int r = (int)(t % R);
+  const long ts = t / R;
+  const int site = (int)(ts % a.N);
+  const int node = (int)(ts / a.N);
   const size_t row = (size_t)node * a.N + site;
-  double d[RS], p[RS];
+  double d[S], p[S];
 #pragma unroll
-  for (int k = 0; k < RS; k++) { d[k] = a.darr[row * RS + k]; p[k] = a.parr[row * RS + k]; }
+  for (int k = 0; k < S; k++) { d[k] = a.darr[row * RS + r * S + k]; p[k] = a.parr[row * RS + r * S + k]; }
   int ed = a.darr_e[row], ep = a.parr_e[row];
-  pull<RS, S>(d, ed, -10);
-  pull<RS, S>(p, ep, -10);
+  pull<R, S>(d, ed, -10);
+  pull<R, S>(p, ep, -10);
 #pragma unroll
-  for (int k = 0; k < RS; k++) {
-    a.darr[row * RS + k] = d[k];
-    a.parr[row * RS + k] = p[k];
+  for (int k = 0; k < S; k++) {
+    a.darr[row * RS + r * S + k] = d[k];
+    a.parr[row * RS + r * S + k] = p[k];
   }
-  a.darr_e[row] = ed;
-  a.parr_e[row] = ep;
+  if (r == 0) {
+    a.darr_e[row] = ed;
+    a.parr_e[row] = ep;
+  }
   if (node < E) {
-    double ud[RS], up[RS];
-    const double *Ph = a.Phalf + (size_t)node * RS * S;
-    evolve_row<RS, S>(Ph, d, ud);
-    evolve_row<RS, S>(Ph, p, up);
+    double ud[S], up[S];
+    const double *Ph = a.Phalf + ((size_t)node * R + r) * S * S;
+    evolve_rate<S>(Ph, d, ud);
+    evolve_rate<S>(Ph, p, up);
     const size_t srow = (size_t)site * Epad + node;
 #pragma unroll
-    for (int k = 0; k < RS; k++) sn64[srow * RS + k] = (statd[k % S] * ud[k]) * up[k];
-    sn_e[srow] = ed + ep;
+    for (int k = 0; k < S; k++) sn64[srow * RS + r * S + k] = (statd[k] * ud[k]) * up[k];
+    if (r == 0) sn_e[srow] = ed + ep;
   }
 }
 
