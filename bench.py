@@ -28,6 +28,17 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
 FP32_PEAK_TFLOPS = 157.3
 
 
+WORKLOAD_TEXT = {
+    "gtr2000": "16S-scale GTR+G4: 2000-taxon Yule tree (3997 edges), 1600 columns, %d queries, "
+               "masked length 100-500, 2%% substitutions",
+    "prot500": "protein 20-state empirical matrix +G4: 500-taxon Yule tree (997 edges), 400 columns, "
+               "%d queries, masked length 60-400, 2%% substitutions",
+    "gtr10000": "GTR+G4 stream shape: 10000-taxon Yule tree (19997 edges), 1500 columns, %d queries, "
+                "masked length 80-600",
+    "gtr50": "GTR+G4: 50-taxon Yule tree (97 edges), 1000 columns, %d queries, fragments 150-300",
+}
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -147,6 +158,7 @@ def main():
     pl, queries, statd, rates = placer.setup_from_workload(wl)
     E = wl.tree.n_nodes - 1
     N = len(queries[0])
+    S = wl.n_states
     log("tree ready in %.1f s (G=%d, N=%d); uploading queries" % (time.perf_counter() - t0,
                                                                   wl.tree.n_nodes, N))
     pl.upload_queries(queries)
@@ -181,7 +193,7 @@ def main():
         st = stats_l[-1]
         ms = 1000.0 * sum(times) / len(times)
         w = 8 if f == "fp64" else 4
-        s1_ref_flops, s1_gather_bytes, s2_flops = kernel_work(st, w=w)
+        s1_ref_flops, s1_gather_bytes, s2_flops = kernel_work(st, S=S, w=w)
         forms[f] = {
             "ms_per_step": ms,
             "value": st.n_logdots / (ms / 1000.0),
@@ -191,7 +203,7 @@ def main():
             "stage1_ref_equiv_tflops": s1_ref_flops / (st.t_stage1_ms * 1e-3) / 1e12,
             "stage1_gather_GBps": s1_gather_bytes / (st.t_stage1_ms * 1e-3) / 1e9,
             "stage2_tflops": s2_flops / ((st.t_pitch_ms + st.t_refine_ms) * 1e-3) / 1e12,
-            "pitch_u2_tflops": pitch_flops(st) / (st.t_pitch_ms * 1e-3) / 1e12,
+            "pitch_u2_tflops": pitch_flops(st, S=S) / (st.t_pitch_ms * 1e-3) / 1e12,
             "pitch_launches": st.n_pitch_launches,
             "pitch_avg_launch_ms": st.t_pitch_ms / max(1, st.n_pitch_launches),
             "n_logdots": st.n_logdots, "n_like_calls": st.n_like_calls,
@@ -202,7 +214,7 @@ def main():
     # dominant kernel: the one with the largest device time
     kern = max([("stage1_kernel", st_main["t_stage1_ms"]), ("pitch_kernel", st_main["t_pitch_ms"]),
                 ("refine_kernels", st_main["t_refine_ms"])], key=lambda kv: kv[1])[0]
-    peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
+    peak = FP64_PEAK_TFLOPS if (args.precision == "fp64" or S == 20) else FP32_PEAK_TFLOPS
     real = "double" if args.precision == "fp64" else "float"
     if kern == "stage1_kernel":
         name = "stage1_kernel<%s,16>" % real
@@ -212,10 +224,13 @@ def main():
                     "traffic": read_traffic(name),
                     "note": "gathered table bytes; most are served by L1/L2"}
     else:
-        name = "pitch_mw_kernel<%s,4,512,2>" % real
+        # the 20-state form runs stage 2 in FP64 in both forms
+        name = ("pitch_mw_kernel<%s,4,512,2>" % real if S == 4
+                else "pitch_prot_kernel<double,4>")
         achieved = st_main["pitch_u2_tflops"]
         roofline = {"bound": "mfma",
-                    "pipe": "vector ALU %s (4-state contractions do not use MFMA)" % args.precision,
+                    "pipe": ("vector ALU %s (4-state contractions do not use MFMA)" % args.precision
+                             if S == 4 else "vector ALU fp64 (eigenbasis evolves)"),
                     "kernel": name, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                     "frac": achieved / peak, "traffic": read_traffic(name),
                     "avg_launch_ms": st_main["pitch_avg_launch_ms"],
@@ -238,8 +253,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic (seeded simulation)",
-        "config": {"workload": "16S-scale GTR+G4: 2000-taxon Yule tree (3997 edges), 1600 columns, "
-                               "%d queries, masked length 100-500, 2%% substitutions" % args.queries,
+        "config": {"workload": WORKLOAD_TEXT.get(args.workload, args.workload + ", %d queries") % args.queries,
                    "queries": args.queries, "edges": E, "columns": N,
                    "form": "plain FP64" if args.precision == "fp64" else "batched FP32"},
         "queries_per_s": main_f["queries_per_s"],
