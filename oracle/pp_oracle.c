@@ -569,12 +569,27 @@ int or_premask(int n_states, const char *ref_seqs, int n_ref, const char *query_
 /* gmix_model.ml GLV operations                                              */
 /* ======================================================================== */
 
-/* gmix_model.ml:308-315 -> gstar_support.ml:70-79 -> linear_c.c:29-90 */
+/* transition matrices one evolve needs: one per rate, or one per CAT category */
+static int n_pslices(const or_model *m) { return m->n_cats > 0 ? m->n_cats : m->n_rates; }
+
+/* gmix_model.ml:308-315 -> gstar_support.ml:70-79 -> linear_c.c:29-90;
+ * CAT: gcat_model.ml:327-345 (per-site P by category through reind, the
+ * masked-to-original site map of core.ml:25-28; NULL = identity), each site a
+ * dgemv with beta = 0, i.e. the same sequential sum over j as gemmish_c. */
 static void evolve_with(const or_model *m, double *tensor, double *dst_a, int64_t *dst_e,
-                        const double *src_a, const int64_t *src_e, int n_sites, double bl)
+                        const double *src_a, const int64_t *src_e, int n_sites, double bl,
+                        const int *reind)
 {
   int R = m->n_rates, S = m->n_states;
   memcpy(dst_e, src_e, sizeof(int64_t) * n_sites);
+  if (m->n_cats > 0) {
+    or_multi_exp(tensor, m->x, m->l, m->xit, m->cat_rates, m->n_cats, S, bl);
+    for (int i = 0; i < n_sites; i++) {
+      int cat = m->site_cat[reind ? reind[i] : i];
+      or_gemmish(dst_a + (long)i * S, tensor + (long)cat * S * S, src_a + (long)i * S, 1, S);
+    }
+    return;
+  }
   or_multi_exp(tensor, m->x, m->l, m->xit, m->rates, R, S, bl);
   for (int r = 0; r < R; r++)
     or_gemmish(dst_a + (long)r * n_sites * S, tensor + (long)r * S * S,
@@ -584,8 +599,8 @@ static void evolve_with(const or_model *m, double *tensor, double *dst_a, int64_
 void or_evolve_into(const or_model *m, double *dst_a, int64_t *dst_e,
                     const double *src_a, const int64_t *src_e, int n_sites, double bl)
 {
-  double *tensor = malloc(sizeof(double) * m->n_rates * m->n_states * m->n_states);
-  evolve_with(m, tensor, dst_a, dst_e, src_a, src_e, n_sites, bl);
+  double *tensor = malloc(sizeof(double) * n_pslices(m) * m->n_states * m->n_states);
+  evolve_with(m, tensor, dst_a, dst_e, src_a, src_e, n_sites, bl, NULL);
   free(tensor);
 }
 
@@ -695,7 +710,7 @@ int or_calc_distal_and_proximal(const or_model *m, const or_tree *t,
 {
   int R = m->n_rates, S = m->n_states, G = t->n_nodes;
   long L = (long)R * n_sites * S;
-  double *tensor = malloc(sizeof(double) * R * S * S);
+  double *tensor = malloc(sizeof(double) * n_pslices(m) * S * S);
   glv *xs = malloc(sizeof(glv) * (G + 1));
   int st = OR_OK;
   /* post-order: ids are post-order, so increasing id visits children first */
@@ -710,7 +725,7 @@ int or_calc_distal_and_proximal(const or_model *m, const or_tree *t,
       st = prep_constant_rate(distal, leaf_seqs + (long)id * n_sites, R, n_sites, S);
     }
     glv ev = glv_at(util_arr, util_arr_e, id, L, n_sites);
-    evolve_with(m, tensor, ev.a, ev.e, distal.a, distal.e, n_sites, t->bl[id]);
+    evolve_with(m, tensor, ev.a, ev.e, distal.a, distal.e, n_sites, t->bl[id], NULL);
   }
   if (st == OR_OK) {
     /* pre-order: root proximal is the unit GLV */
@@ -722,7 +737,8 @@ int or_calc_distal_and_proximal(const or_model *m, const or_tree *t,
       int c0 = t->child_start[id], c1 = t->child_start[id + 1];
       if (c1 == c0) continue;
       glv prox = glv_at(parr, parr_e, id, L, n_sites);
-      evolve_with(m, tensor, evolved_prox.a, evolved_prox.e, prox.a, prox.e, n_sites, t->bl[id]);
+      evolve_with(m, tensor, evolved_prox.a, evolved_prox.e, prox.a, prox.e, n_sites, t->bl[id],
+                  NULL);
       for (int k = c0; k < c1; k++) {  /* ListFuns.pull_each_out: rest keeps order */
         int n = 0;
         xs[n++] = evolved_prox;
@@ -757,13 +773,13 @@ void or_prep_supernodes(const or_model *m, const or_tree *t, int n_sites,
 {
   int R = m->n_rates, S = m->n_states, G = t->n_nodes;
   long L = (long)R * n_sites * S;
-  double *tensor = malloc(sizeof(double) * R * S * S);
+  double *tensor = malloc(sizeof(double) * n_pslices(m) * S * S);
   double *ud = malloc(sizeof(double) * L), *up = malloc(sizeof(double) * L);
   int64_t *ude = malloc(sizeof(int64_t) * n_sites), *upe = malloc(sizeof(int64_t) * n_sites);
   for (int i = 0; i < G; i++) {
     double half = t->bl[i] / 2.;
-    evolve_with(m, tensor, ud, ude, darr + i * L, darr_e + (long)i * n_sites, n_sites, half);
-    evolve_with(m, tensor, up, upe, parr + i * L, parr_e + (long)i * n_sites, n_sites, half);
+    evolve_with(m, tensor, ud, ude, darr + i * L, darr_e + (long)i * n_sites, n_sites, half, NULL);
+    evolve_with(m, tensor, up, upe, parr + i * L, parr_e + (long)i * n_sites, n_sites, half, NULL);
     for (int k = 0; k < n_sites; k++) snodes_e[(long)i * n_sites + k] = ude[k] + upe[k];
     or_ten_statd_pairwise_prod(m->statd, snodes + i * L, ud, up, R, n_sites, S);
   }
@@ -874,12 +890,13 @@ typedef struct {
   glv prox_o, dist_o, pend_o, prox_v, dist_v, pend_v;
   double prox_bl, dist_bl, pend_bl;
   long n_like_calls;
+  int *reind;         /* masked site -> pre-masked site (CAT category lookup) */
 } three_tax;
 
 static void tt_set_edge_bl(three_tax *tt, glv orig, glv evolv, double *blp, double bl)
 {
   *blp = bl;   /* glv_edge.ml:29-44 set_bl -> recalculate */
-  evolve_with(tt->m, tt->tensor, evolv.a, evolv.e, orig.a, orig.e, tt->M, bl);
+  evolve_with(tt->m, tt->tensor, evolv.a, evolv.e, orig.a, orig.e, tt->M, bl, tt->reind);
 }
 
 /* three_tax.ml:30-36 */
@@ -1009,8 +1026,9 @@ static int alloc_tt(three_tax *tt, const or_model *m, int M)
 {
   int R = m->n_rates, S = m->n_states;
   tt->m = m; tt->M = M; tt->L = (long)R * M * S;
-  tt->tensor = malloc(sizeof(double) * R * S * S);
+  tt->tensor = malloc(sizeof(double) * n_pslices(m) * S * S);
   tt->util = malloc(sizeof(double) * (M > 0 ? M : 1));
+  tt->reind = calloc(M > 0 ? M : 1, sizeof(int));
   glv *gs[6] = { &tt->prox_o, &tt->dist_o, &tt->pend_o, &tt->prox_v, &tt->dist_v, &tt->pend_v };
   for (int i = 0; i < 6; i++) {
     gs[i]->a = calloc(tt->L > 0 ? tt->L : 1, sizeof(double));
@@ -1021,7 +1039,7 @@ static int alloc_tt(three_tax *tt, const or_model *m, int M)
 }
 static void free_tt(three_tax *tt)
 {
-  free(tt->tensor); free(tt->util);
+  free(tt->tensor); free(tt->util); free(tt->reind);
   glv *gs[6] = { &tt->prox_o, &tt->dist_o, &tt->pend_o, &tt->prox_v, &tt->dist_v, &tt->pend_v };
   for (int i = 0; i < 6; i++) { free(gs[i]->a); free(gs[i]->e); }
 }
@@ -1092,6 +1110,7 @@ static int setup_query(query_state *qs, const or_core *c, const char *query, uin
   if (M == 0) return OR_BAD_INPUT;   /* "has no informative sites" */
   qs->c = c; qs->mask = mask;
   alloc_tt(&qs->tt, m, M);
+  for (int i = 0, k = 0; i < N; i++) if (mask[i]) qs->tt.reind[k++] = i;
   /* query_glv: constant-rate GLV of the masked characters (core.ml:166-170) */
   char *mq = malloc(M);
   for (int i = 0, k = 0; i < N; i++) if (mask[i]) mq[k++] = query[i];

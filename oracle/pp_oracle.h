@@ -94,6 +94,12 @@ typedef struct {
   const double *l;      /* [S]   */
   const double *xit;    /* [S][S] */
   const double *rates;  /* [R]   */
+  /* Price-CAT (gcat_model.ml:33-95, 327-345): with n_cats > 0 the GLVs carry
+   * no rate axis (n_rates == 1, rates = {1}) and site i evolves by the
+   * matrix of rate cat_rates[site_cat[i]] (i indexes the pre-masked sites). */
+  int n_cats;
+  const double *cat_rates;  /* [n_cats] */
+  const int *site_cat;      /* [N] */
 } or_model;
 
 void or_evolve_into(const or_model *m, double *dst_a, int64_t *dst_e,

@@ -29,7 +29,8 @@ _ip = C.POINTER(C.c_int)
 
 class OrModel(C.Structure):
     _fields_ = [("n_rates", C.c_int), ("n_states", C.c_int), ("statd", _dp), ("x", _dp),
-                ("l", _dp), ("xit", _dp), ("rates", _dp)]
+                ("l", _dp), ("xit", _dp), ("rates", _dp), ("n_cats", C.c_int),
+                ("cat_rates", _dp), ("site_cat", _ip)]
 
 
 class OrTree(C.Structure):
@@ -310,17 +311,29 @@ def premask(n_states, ref_seqs, query_seqs, discard_nonoverlapped=False):
 
 
 class Model:
-    """Gamma mixture model: statd, rates and the diagonalised Q (x, l, xit)."""
+    """Gamma mixture model: statd, rates and the diagonalised Q (x, l, xit).
+    With cat_rates and site_cat it is the Price-CAT model (gcat_model.ml):
+    rate-free GLVs, site i evolving at rate cat_rates[site_cat[i]] (site_cat
+    indexes the pre-masked sites)."""
 
-    def __init__(self, exch, statd, rates):
+    def __init__(self, exch, statd, rates=None, cat_rates=None, site_cat=None):
         self.exch = f64(exch)
         self.statd = f64(statd)
-        self.rates = f64(rates)
         self.x, self.l, self.xit = normed_of_exchangeable_pair(self.exch, self.statd)
         self.S = self.statd.shape[0]
+        if cat_rates is not None:
+            self.rates = f64([1.0])
+            self.cat_rates = f64(cat_rates)
+            self.site_cat = np.ascontiguousarray(site_cat, dtype=np.int32)
+            assert self.site_cat.min() >= 0 and self.site_cat.max() < len(self.cat_rates)
+            n_cats, cr, sc = len(self.cat_rates), P(self.cat_rates), P(self.site_cat, C.c_int)
+        else:
+            self.rates = f64(rates)
+            self.cat_rates = self.site_cat = None
+            n_cats, cr, sc = 0, None, None
         self.R = self.rates.shape[0]
         self.c = OrModel(self.R, self.S, P(self.statd), P(self.x), P(self.l), P(self.xit),
-                         P(self.rates))
+                         P(self.rates), n_cats, cr, sc)
 
     def evolve(self, a, e, bl):
         a = f64(a)

@@ -115,3 +115,27 @@ def placement_loglike(tree, leaf_seqs, exch, statd, rates, loc, dist, pend, quer
     sites = [k for k, c in enumerate(query) if c not in "-?"]
     root = attach(tree, leaf_seqs, loc, dist, pend, query)
     return float(np.sum(site_loglikes(root, exch, statd, rates, sites)))
+
+
+def site_loglikes_cat(root, exch, statd, cat_rates, site_cat, sites=None):
+    """Price-CAT site log-likelihoods: site k evolves at rate
+    cat_rates[site_cat[k]] alone (no mixture, no 1/R)."""
+    site_cat = np.asarray(site_cat)
+    sites = np.arange(len(site_cat)) if sites is None else np.asarray(sites)
+    out = np.zeros(len(sites))
+    for c in np.unique(site_cat[sites]):
+        pick = np.nonzero(site_cat[sites] == c)[0]
+        out[pick] = site_loglikes(root, exch, statd, [cat_rates[c]], sites[pick])
+    return out
+
+
+def tree_loglike_cat(tree, leaf_seqs, exch, statd, cat_rates, site_cat):
+    root = from_tree(tree, leaf_seqs)[tree.root]
+    return float(np.sum(site_loglikes_cat(root, exch, statd, cat_rates, site_cat)))
+
+
+def placement_loglike_cat(tree, leaf_seqs, exch, statd, cat_rates, site_cat, loc, dist, pend,
+                          query):
+    sites = [k for k, c in enumerate(query) if c not in "-?"]
+    root = attach(tree, leaf_seqs, loc, dist, pend, query)
+    return float(np.sum(site_loglikes_cat(root, exch, statd, cat_rates, site_cat, sites)))
