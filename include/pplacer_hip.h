@@ -108,6 +108,15 @@ int pp_model_build(pp_ctx *ctx, int n_states, int n_rates, const double *exch,
 /* read back the diagonalisation: x [S][S], l [S], xit [S][S] */
 int pp_model_get(pp_ctx *ctx, double *x, double *l, double *xit);
 
+/* Price-CAT model (gcat_model.ml:33-95, refpkg.ml:39-63 "Price-CAT" block):
+ * rate-free GLVs, each site evolving by the matrix of its category's rate.
+ * The categories are those of the pre-masked sites (Gcat_model.mask_sites,
+ * gcat_model.ml:429-439, after the 0-based shift of init_of_json :475) and
+ * must be set before pp_tree_set / pp_tree_upload. */
+int pp_model_build_cat(pp_ctx *ctx, int n_states, int n_cats, const double *exch,
+                       const double *statd, const double *cat_rates);
+int pp_site_categories(pp_ctx *ctx, int n_sites, const int *site_cat);
+
 /* Gamma.discrete_gamma (gamma.ml:52-70) */
 int pp_discrete_gamma(int n_cat, double alpha, double *rates);
 

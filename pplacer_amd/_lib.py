@@ -67,6 +67,8 @@ SIGNATURES = {
     "pp_ctx_destroy": (_i, [_vp]),
     "pp_model_build": (_i, [_vp, _i, _i, _dp, _dp, _dp]),
     "pp_model_get": (_i, [_vp, _dp, _dp, _dp]),
+    "pp_model_build_cat": (_i, [_vp, _i, _i, _dp, _dp, _dp]),
+    "pp_site_categories": (_i, [_vp, _i, C.POINTER(C.c_int)]),
     "pp_discrete_gamma": (_i, [_i, _d, _dp]),
     "pp_emper_freq": (_i, [_i, C.c_char_p, _i, _i, _dp]),
     "pp_premask": (_i, [_i, C.c_char_p, _i, C.c_char_p, _i, _i, _i, _u8p, _u8p, _ip]),

@@ -95,6 +95,7 @@ struct pp_ctx {
   DBuf b_de32, b_pe32;  // exponents of the FP32 copies
   DBuf b_lt;            // stage-1 log-util table [N][n_inf+1][Epad]
   DBuf b_scratch;       // 20-state stage-2 pair scratch
+  DBuf b_cat;           // Price-CAT: category of each pre-masked site [N]
   std::vector<double> h_ttab;
   // resident queries (sorted by first informative column)
   int Q = 0;
@@ -119,16 +120,21 @@ int set_device(pp_ctx *c) {
 
 // stage-1 table: every character's LV evolved by start_pend (the rows of
 // full_query_evolv, core.ml:137-141), [n_codes][R*S], FP64 as the stubs do
+// [n_tsl][n_codes][R*S]: n_tsl = 1 for the gamma model (rate r uses matrix
+// r); with Price-CAT one table per category (R == 1).
 std::vector<double> query_table(const HostModel &m, double bl) {
-  const int R = m.R, S = m.S;
-  std::vector<double> P(R * S * S), T(m.n_codes * R * S);
+  const int R = m.R, S = m.S, n_sl = m.n_sl(), n_tsl = m.n_cats > 0 ? m.n_cats : 1;
+  std::vector<double> P((size_t)n_sl * S * S), T((size_t)n_tsl * m.n_codes * R * S);
   multi_exp(m, bl, P.data());
-  for (int c = 0; c < m.n_codes; c++)
-    for (int r = 0; r < R; r++)
-      for (int i = 0; i < S; i++) {
-        double acc = 0;
-        for (int j = 0; j < S; j++) acc += P[r * S * S + i * S + j] * m.lv[c * S + j];
-        T[(c * R + r) * S + i] = acc;
+  for (int tsl = 0; tsl < n_tsl; tsl++)
+    for (int c = 0; c < m.n_codes; c++)
+      for (int r = 0; r < R; r++) {
+        const double *Pr = P.data() + (size_t)(m.n_cats > 0 ? tsl : r) * S * S;
+        for (int i = 0; i < S; i++) {
+          double acc = 0;
+          for (int j = 0; j < S; j++) acc += Pr[i * S + j] * m.lv[c * S + j];
+          T[(((size_t)tsl * m.n_codes + c) * R + r) * S + i] = acc;
+        }
       }
   return T;
 }
@@ -154,9 +160,27 @@ void to_device_layout(const TreeGlvs &g, int G, int N, int R, int S, int Epad,
 
 int make_fp32_copies(pp_ctx *c);
 
+// Price-CAT: the site categories on the device (nullptr for the gamma model);
+// upload_site_cat checks them against the tree's site count.
+const int *site_cat_ptr(const pp_ctx *c) {
+  return c->model.n_cats > 0 ? reinterpret_cast<const int *>(c->b_cat.p) : nullptr;
+}
+
+int upload_site_cat(pp_ctx *c, int N) {
+  if (c->model.n_cats == 0) return PP_OK;
+  CHECK((int)c->model.site_cat.size() == N, PP_ESTATE,
+        "Price-CAT model: pp_site_categories must give one category per site");
+  int *d;
+  ENSURE(c->b_cat, std::max(N, 1), &d);
+  HIPCHECK(hipMemcpy(d, c->model.site_cat.data(), (size_t)N * 4, hipMemcpyHostToDevice));
+  return PP_OK;
+}
+
 int upload_glvs(pp_ctx *c, const TreeGlvs &g) {
   const int G = c->G, N = c->N, R = c->model.R, S = c->model.S, Epad = c->Epad;
   if (S != 4 && S != 20) { set_error("the GPU placement path supports n_states 4 and 20"); return PP_EINVAL; }
+  const int st_cat = upload_site_cat(c, N);
+  if (st_cat) return st_cat;
   const size_t glv = (size_t)G * N * R * S, snn = (size_t)N * Epad * R * S;
   double *d64, *p64, *s64;
   int *de, *pe, *se;
@@ -235,8 +259,9 @@ int gpu_precompute_t(pp_ctx *c, const TreeDesc &t, const std::vector<uint8_t> &c
   int *evde;
   ENSURE(b_evd, glv, &evd);
   ENSURE(b_evde, (size_t)G * N, &evde);
-  ENSURE(b_P, (size_t)G * RS * S, &P);
-  ENSURE(b_Ph, (size_t)G * RS * S, &Ph);
+  const int n_sl = c->model.n_sl();
+  ENSURE(b_P, (size_t)G * n_sl * S * S, &P);
+  ENSURE(b_Ph, (size_t)G * n_sl * S * S, &Ph);
   ENSURE(b_cs, G + 1, &cs);
   ENSURE(b_ch, std::max(1, t.child_start[G]), &ch);
   ENSURE(b_par, G, &par);
@@ -245,10 +270,10 @@ int gpu_precompute_t(pp_ctx *c, const TreeDesc &t, const std::vector<uint8_t> &c
   ENSURE(b_lv, c->model.lv.size(), &lv);
   ENSURE(b_sd, S, &statd);
   // transition matrices P(bl) and P(bl/2) for every node, FP64 on the host
-  std::vector<double> hP((size_t)G * RS * S), hPh((size_t)G * RS * S);
+  std::vector<double> hP((size_t)G * n_sl * S * S), hPh((size_t)G * n_sl * S * S);
   for (int i = 0; i < G; i++) {
-    multi_exp(c->model, t.bl[i], hP.data() + (size_t)i * RS * S);
-    multi_exp(c->model, t.bl[i] / 2., hPh.data() + (size_t)i * RS * S);
+    multi_exp(c->model, t.bl[i], hP.data() + (size_t)i * n_sl * S * S);
+    multi_exp(c->model, t.bl[i] / 2., hPh.data() + (size_t)i * n_sl * S * S);
   }
   HIPCHECK(hipMemcpy(P, hP.data(), hP.size() * 8, hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(Ph, hPh.data(), hPh.size() * 8, hipMemcpyHostToDevice));
@@ -266,6 +291,8 @@ int gpu_precompute_t(pp_ctx *c, const TreeDesc &t, const std::vector<uint8_t> &c
   a.parent = par;
   a.P = P;
   a.Phalf = Ph;
+  a.site_cat = site_cat_ptr(c);
+  a.n_sl = n_sl;
   a.leaf_codes = lc;
   a.lv = lv;
   a.darr = d64;
@@ -354,6 +381,8 @@ int gpu_precompute(pp_ctx *c, const TreeDesc &t, const char *leaf_seqs) {
       codes[(size_t)id * N + k] = (uint8_t)code;
     }
   }
+  const int st_cat = upload_site_cat(c, N);
+  if (st_cat) return st_cat;
   if (S == 4 && R == 4) return gpu_precompute_t<4, 4>(c, t, codes, parent, height, depth);
   if (S == 4 && R == 1) return gpu_precompute_t<1, 4>(c, t, codes, parent, height, depth);
   if (S == 20 && R == 4) return gpu_precompute_t<4, 20>(c, t, codes, parent, height, depth);
@@ -385,6 +414,8 @@ int build_code_logs(pp_ctx *c, double start_pend) {
   a.Epad = Epad;
   a.n_inf = n_inf;
   a.ttab = d_ttab;
+  a.n_codes = c->model.n_codes;
+  a.site_cat = site_cat_ptr(c);
   a.lt64 = std::is_same<Real, double>::value ? reinterpret_cast<double *>(lt) : nullptr;
   a.lt32 = std::is_same<Real, float>::value ? reinterpret_cast<float *>(lt) : nullptr;
   const dim3 grid(Epad / 256, N);
@@ -466,8 +497,8 @@ int occupancy_grid(const void *kern, int bd, size_t lds, int n_cu, int n_items) 
 // flags, cached products of the worker lanes' sites.
 template <typename Real, int R>
 size_t stage2_lds(const pp_ctx *c, int bd, int taken_bytes) {
-  using L = MWLayout<Real, R>;
-  return L::align(L::bytes(c->N) + (size_t)taken_bytes) + MWProducts<Real, R * 4>::bytes((bd - 64) * kSPL);
+  const MWLayout<Real, R> lay(c->model.n_cats > 0 ? c->model.n_cats : 1, c->model.n_sl());
+  return lay.align(lay.bytes(c->N) + (size_t)taken_bytes) + MWProducts<Real, R * 4>::bytes((bd - 64) * kSPL);
 }
 
 // Whether the FP32 form runs stage 2 in FP32.  The 20-state engine evaluates
@@ -483,44 +514,46 @@ int max_masked_sites(const pp_ctx *c) {
   if (c->model.S == 20) return c->N;  // pair state lives in global scratch
   int m = 0;
   for (int b = 0; b < kBuckets; b++) {
-    if (stage2_lds<double, 4>(c, block_of_bucket(b), c->E) > c->max_lds) break;
+    const size_t lds = c->model.R == 4 ? stage2_lds<double, 4>(c, block_of_bucket(b), c->E)
+                                       : stage2_lds<double, 1>(c, block_of_bucket(b), c->E);
+    if (lds > c->max_lds) break;
     m = 64 * kSPL * (b + 1);
   }
   return m;
 }
 
-template <typename Real, int R, int MAXBD>
+template <typename Real, int R, int MAXBD, bool CAT>
 int launch_pitch_b(pp_ctx *c, PitchArgs a, int bd) {
   const size_t lds = stage2_lds<Real, R>(c, bd, c->E);
-  const void *k = reinterpret_cast<const void *>(&pitch_mw_kernel<Real, R, MAXBD, kSPL>);
+  const void *k = reinterpret_cast<const void *>(&pitch_mw_kernel<Real, R, MAXBD, kSPL, CAT>);
   const int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.nq);
-  hipLaunchKernelGGL((pitch_mw_kernel<Real, R, MAXBD, kSPL>), dim3(grid), dim3(bd), lds, c->stream, a);
+  hipLaunchKernelGGL((pitch_mw_kernel<Real, R, MAXBD, kSPL, CAT>), dim3(grid), dim3(bd), lds, c->stream, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
 
-template <typename Real, int R, int MAXBD>
+template <typename Real, int R, int MAXBD, bool CAT>
 int launch_opt_b(pp_ctx *c, OptArgs a, int bd) {
   const size_t lds = stage2_lds<Real, R>(c, bd, 0);
-  const void *k = reinterpret_cast<const void *>(&optimize_mw_kernel<Real, R, MAXBD, kSPL>);
+  const void *k = reinterpret_cast<const void *>(&optimize_mw_kernel<Real, R, MAXBD, kSPL, CAT>);
   const int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.n_jobs);
-  hipLaunchKernelGGL((optimize_mw_kernel<Real, R, MAXBD, kSPL>), dim3(grid), dim3(bd), lds, c->stream, a);
+  hipLaunchKernelGGL((optimize_mw_kernel<Real, R, MAXBD, kSPL, CAT>), dim3(grid), dim3(bd), lds, c->stream, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
 
-template <typename Real, int R>
+template <typename Real, int R, bool CAT = false>
 int launch_pitch(pp_ctx *c, PitchArgs a, int bd) {
-  if (bd <= 512) return launch_pitch_b<Real, R, 512>(c, a, bd);
-  if (bd <= 768) return launch_pitch_b<Real, R, 768>(c, a, bd);
-  return launch_pitch_b<Real, R, 1024>(c, a, bd);
+  if (bd <= 512) return launch_pitch_b<Real, R, 512, CAT>(c, a, bd);
+  if (bd <= 768) return launch_pitch_b<Real, R, 768, CAT>(c, a, bd);
+  return launch_pitch_b<Real, R, 1024, CAT>(c, a, bd);
 }
 
-template <typename Real, int R>
+template <typename Real, int R, bool CAT = false>
 int launch_opt(pp_ctx *c, OptArgs a, int bd) {
-  if (bd <= 512) return launch_opt_b<Real, R, 512>(c, a, bd);
-  if (bd <= 768) return launch_opt_b<Real, R, 768>(c, a, bd);
-  return launch_opt_b<Real, R, 1024>(c, a, bd);
+  if (bd <= 512) return launch_opt_b<Real, R, 512, CAT>(c, a, bd);
+  if (bd <= 768) return launch_opt_b<Real, R, 768, CAT>(c, a, bd);
+  return launch_opt_b<Real, R, 1024, CAT>(c, a, bd);
 }
 
 // 20-state stage 2: one master + 7 worker waves per query, pairs per lane as
@@ -573,12 +606,14 @@ int launch_opt_prot(pp_ctx *c, OptArgs a) {
 
 template <typename Real>
 int dispatch_pitch(pp_ctx *c, PitchArgs a, int bd) {
+  a.site_cat = site_cat_ptr(c);
   if (c->model.S == 20) {
     if (c->model.R == 4) return launch_pitch_prot<Real, 4>(c, a);
     if (c->model.R == 1) return launch_pitch_prot<Real, 1>(c, a);
     set_error("n_rates must be 1 or 4 on the GPU path");
     return PP_EINVAL;
   }
+  if (c->model.n_cats > 0) return launch_pitch<Real, 1, true>(c, a, bd);
   if (c->model.R == 4) return launch_pitch<Real, 4>(c, a, bd);
   if (c->model.R == 1) return launch_pitch<Real, 1>(c, a, bd);
   set_error("n_rates must be 1 or 4 on the GPU path");
@@ -586,12 +621,14 @@ int dispatch_pitch(pp_ctx *c, PitchArgs a, int bd) {
 }
 template <typename Real>
 int dispatch_opt(pp_ctx *c, OptArgs a, int bd) {
+  a.site_cat = site_cat_ptr(c);
   if (c->model.S == 20) {
     if (c->model.R == 4) return launch_opt_prot<Real, 4>(c, a);
     if (c->model.R == 1) return launch_opt_prot<Real, 1>(c, a);
     set_error("n_rates must be 1 or 4 on the GPU path");
     return PP_EINVAL;
   }
+  if (c->model.n_cats > 0) return launch_opt<Real, 1, true>(c, a, bd);
   if (c->model.R == 4) return launch_opt<Real, 4>(c, a, bd);
   if (c->model.R == 1) return launch_opt<Real, 1>(c, a, bd);
   set_error("n_rates must be 1 or 4 on the GPU path");
@@ -913,7 +950,7 @@ int pp_ctx_destroy(pp_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   DBuf *bufs[] = {&c->b_model, &c->b_bl, &c->b_d64, &c->b_p64, &c->b_d32, &c->b_p32, &c->b_de,
-                  &c->b_pe, &c->b_s64, &c->b_se, &c->b_de32, &c->b_pe32, &c->b_lt, &c->b_scratch, &c->b_codes, &c->b_qM, &c->b_tcodes,
+                  &c->b_pe, &c->b_s64, &c->b_se, &c->b_de32, &c->b_pe32, &c->b_lt, &c->b_scratch, &c->b_cat, &c->b_codes, &c->b_qM, &c->b_tcodes,
                   &c->b_tile_lo, &c->b_tile_hi, &c->b_tile_q, &c->b_ttab, &c->b_scores, &c->b_res,
                   &c->b_nres, &c->b_status, &c->b_kept, &c->b_nkeep, &c->b_out, &c->b_nout,
                   &c->b_qlist, &c->b_counter, &c->b_stats};
@@ -921,6 +958,17 @@ int pp_ctx_destroy(pp_ctx *c) {
     if (b->p) (void)hipFree(b->p);
   (void)hipStreamDestroy(c->stream);
   delete c;
+  return PP_OK;
+}
+
+static int upload_model(pp_ctx *c) {
+  DevModel dm;
+  c->model.fill_dev(&dm);
+  DevModel *d;
+  ENSURE(c->b_model, 1, &d);
+  HIPCHECK(hipMemcpy(d, &dm, sizeof(dm), hipMemcpyHostToDevice));
+  c->d_model = d;
+  c->has_tree = false;
   return PP_OK;
 }
 
@@ -932,12 +980,31 @@ int pp_model_build(pp_ctx *c, int n_states, int n_rates, const double *exch, con
   std::string err;
   st = build_model(c->model, n_states, n_rates, exch, statd, rates, err);
   if (st) { set_error(err); return st; }
-  DevModel dm;
-  c->model.fill_dev(&dm);
-  DevModel *d;
-  ENSURE(c->b_model, 1, &d);
-  HIPCHECK(hipMemcpy(d, &dm, sizeof(dm), hipMemcpyHostToDevice));
-  c->d_model = d;
+  return upload_model(c);
+}
+
+int pp_model_build_cat(pp_ctx *c, int n_states, int n_cats, const double *exch, const double *statd,
+                       const double *cat_rates) {
+  CHECK(c && exch && statd && cat_rates, PP_EINVAL, "null argument");
+  CHECK(n_cats >= 1 && n_cats <= kMaxRates, PP_EINVAL, "n_cats must be in 1..32");
+  for (int k = 0; k < n_cats; k++) CHECK(cat_rates[k] >= 0., PP_EINVAL, "negative category rate");
+  int st = set_device(c);
+  if (st) return st;
+  std::string err;
+  const double one = 1.0;
+  st = build_model(c->model, n_states, 1, exch, statd, &one, err);
+  if (st) { set_error(err); return st; }
+  c->model.n_cats = n_cats;
+  c->model.cat_rates.assign(cat_rates, cat_rates + n_cats);
+  return upload_model(c);
+}
+
+int pp_site_categories(pp_ctx *c, int n_sites, const int *site_cat) {
+  CHECK(c && site_cat && n_sites >= 0, PP_EINVAL, "bad arguments");
+  CHECK(c->model.built && c->model.n_cats > 0, PP_ESTATE, "no CAT model: call pp_model_build_cat first");
+  for (int i = 0; i < n_sites; i++)
+    CHECK(site_cat[i] >= 0 && site_cat[i] < c->model.n_cats, PP_EINVAL, "site category out of range");
+  c->model.site_cat.assign(site_cat, site_cat + n_sites);
   c->has_tree = false;
   return PP_OK;
 }

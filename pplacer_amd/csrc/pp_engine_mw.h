@@ -54,20 +54,38 @@ struct MWShared {
   long long ipart[16];       // per-worker-wave sums of site exponents
 };
 
+// LDS of one workgroup: shared header, pendant rows [n_tsl][kMaxCodes][R*4],
+// P(prox) and P(dist) [n_sl][4][4], exponential staging [3][n_sl][4],
+// masked-site list.  Gamma model: n_tsl = 1, n_sl = R (rate r uses slice r);
+// Price-CAT: R = 1 and n_tsl = n_sl = n_cats (a site uses its category's).
 template <typename Real, int R>
 struct MWLayout {
   static constexpr int RS = R * 4;
   static constexpr size_t head = (sizeof(MWShared) + 15) & ~size_t(15);
-  // shared header, pendant rows, P(prox), P(dist), masked-site list
-  static __host__ __device__ size_t bytes(int N) {
-    return head + sizeof(Real) * (kMaxCodes * RS + 2 * R * 16) + sizeof(int) * (size_t)N;
+  int n_tsl, n_sl;
+  __host__ __device__ MWLayout(int n_tsl_, int n_sl_) : n_tsl(n_tsl_), n_sl(n_sl_) {}
+  __host__ __device__ size_t tbytes() const { return sizeof(Real) * (size_t)n_tsl * kMaxCodes * RS; }
+  __host__ __device__ size_t pbytes() const { return sizeof(Real) * (size_t)n_sl * 16; }
+  __host__ __device__ size_t ebytes() const { return sizeof(double) * (size_t)3 * n_sl * 4; }
+  __host__ __device__ size_t bytes(int N) const {
+    return head + tbytes() + 2 * pbytes() + ebytes() + sizeof(int) * (size_t)N;
   }
-  static __device__ Real *T(char *s) { return reinterpret_cast<Real *>(s + head); }
-  static __device__ Real *Pp(char *s) { return T(s) + kMaxCodes * RS; }
-  static __device__ Real *Pd(char *s) { return Pp(s) + R * 16; }
-  static __device__ int *sidx(char *s) { return reinterpret_cast<int *>(Pd(s) + R * 16); }
+  __device__ Real *T(char *s) const { return reinterpret_cast<Real *>(s + head); }
+  __device__ Real *Pp(char *s) const { return reinterpret_cast<Real *>(s + head + tbytes()); }
+  __device__ Real *Pd(char *s) const { return reinterpret_cast<Real *>(s + head + tbytes() + pbytes()); }
+  __device__ double *E(char *s) const {
+    return reinterpret_cast<double *>(s + head + tbytes() + 2 * pbytes());
+  }
+  __device__ int *sidx(char *s) const {
+    return reinterpret_cast<int *>(s + head + tbytes() + 2 * pbytes() + ebytes());
+  }
   static __host__ __device__ size_t align(size_t off) { return (off + 15) & ~size_t(15); }
 };
+
+template <typename Real, int R>
+__device__ __forceinline__ MWLayout<Real, R> mw_layout(const DevModel *dm) {
+  return MWLayout<Real, R>(dm->cat ? dm->n_sl : 1, dm->n_sl);
+}
 
 // Wave-uniform branch on the wave index (wave 0 = master).
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
@@ -121,22 +139,24 @@ struct MWProducts {
   }
 };
 
-template <typename Real, int R, int SPL>
-__device__ __forceinline__ void mw_worker(char *smem, char *wl, int N, const uint8_t *codes,
-                                          const Real *darr, const Real *parr, const int *darr_e,
-                                          const int *parr_e) {
-  using L = MWLayout<Real, R>;
+// CAT: Price-CAT, R == 1, site k uses P slice and pendant table sl[k] = its
+// category (site_cat); otherwise slice r for rate r and pendant table 0.
+template <typename Real, int R, int SPL, bool CAT>
+__device__ __forceinline__ void mw_worker(char *smem, const MWLayout<Real, R> &lay, char *wl, int N,
+                                          const uint8_t *codes, const Real *darr, const Real *parr,
+                                          const int *darr_e, const int *parr_e, const int *site_cat) {
   using Ops = RealOps<Real>;
   constexpr int RS = R * 4;
+  static_assert(!CAT || R == 1, "Price-CAT GLVs have no rate axis");
   using W = MWProducts<Real, RS>;
   MWShared *sh = reinterpret_cast<MWShared *>(smem);
-  const Real *sT = L::T(smem), *sPp = L::Pp(smem), *sPd = L::Pd(smem);
-  const int *sidx = L::sidx(smem);
+  const Real *sT = lay.T(smem), *sPp = lay.Pp(smem), *sPd = lay.Pd(smem);
+  const int *sidx = lay.sidx(smem);
   const int lane = threadIdx.x & 63;
   const int wv = wave_id() - 1;
   const int stride = (int)blockDim.x - 64;  // worker lanes
   const int my = wv * 64 + lane;
-  int site[SPL], slot[SPL], wb[SPL];
+  int site[SPL], slot[SPL], wb[SPL], sl[SPL];
   bool active[SPL];
 #pragma unroll
   for (int k = 0; k < SPL; k++) wb[k] = W::base(my + k * stride);
@@ -152,6 +172,9 @@ __device__ __forceinline__ void mw_worker(char *smem, char *wl, int N, const uin
       active[k] = i < sh->M;
       site[k] = active[k] ? sidx[i] : 0;
       slot[k] = active[k] ? sh->code_slot[codes[(size_t)sh->q * N + site[k]]] : 0;
+      sl[k] = CAT ? site_cat[site[k]] : 0;
+      // pendant row of the site: table sl, character slot
+      slot[k] += sl[k] * kMaxCodes;
     }
     __syncthreads();  // B
     __syncthreads();  // A
@@ -189,6 +212,10 @@ __device__ __forceinline__ void mw_worker(char *smem, char *wl, int N, const uin
               const int t = r * 4 + i;
 #pragma unroll
               for (int k = 0; k < SPL; k++) {
+                if (CAT) {
+                  pp = sPp + sl[k] * 16 + i * 4;
+                  pd = sPd + sl[k] * 16 + i * 4;
+                }
                 Real xv = pp[0] * pO[k][r * 4], yv = pd[0] * dO[k][r * 4];
 #pragma unroll
                 for (int j = 1; j < 4; j++) {
@@ -222,12 +249,20 @@ __device__ __forceinline__ void mw_worker(char *smem, char *wl, int N, const uin
   // op == CMD_EXIT
 }
 
+// Orders one wave's LDS writes before its later LDS reads of other lanes'
+// entries (LDS executes a wave's operations in order).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------
 // master wave
 // ---------------------------------------------------------------------------
-template <typename Real, int R, int SPL>
-struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL>> {
-  using Ctl = ThreeTaxControl<MWMaster<Real, R, SPL>>;
+template <typename Real, int R, int SPL, bool CAT>
+struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT>> {
+  using Ctl = ThreeTaxControl<MWMaster<Real, R, SPL, CAT>>;
   using Ctl::M;
   using Ctl::pend_bl;
   using Ctl::dist_bl;
@@ -244,17 +279,20 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL>> {
   static constexpr int RS = R * 4;
   MWShared *sh;
   Real *sT, *sPp, *sPd;
+  double *sE;
   int *sidx;
-  int lane, n_workers, cap_sites, n_codes, n_inf;
+  int lane, n_workers, cap_sites, n_codes, n_inf, n_sl;
   double log_R;
 
-
   __device__ __forceinline__ void init(char *smem, const DevModel *dm) {
+    const L lay = mw_layout<Real, R>(dm);
     sh = reinterpret_cast<MWShared *>(smem);
-    sT = L::T(smem);
-    sPp = L::Pp(smem);
-    sPd = L::Pd(smem);
-    sidx = L::sidx(smem);
+    sT = lay.T(smem);
+    sPp = lay.Pp(smem);
+    sPd = lay.Pd(smem);
+    sE = lay.E(smem);
+    sidx = lay.sidx(smem);
+    n_sl = dm->n_sl;
     lane = threadIdx.x & 63;
     n_workers = (blockDim.x >> 6) - 1;
     cap_sites = 64 * SPL * n_workers;
@@ -330,6 +368,10 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL>> {
   // Diagd.multi_exp for the changed branch lengths and the evolved pendant
   // rows of the characters present, with the master's 64 lanes.
   __device__ __forceinline__ void tables(bool new_pend, bool new_dist) {
+    if constexpr (CAT) {
+      cat_tables(new_pend, new_dist);
+      return;
+    }
     if (new_dist) {
       // exp(rate_r * bl * l_k) for prox (u < 4R) and dist (4R <= u < 8R)
       double ex = 0.0;
@@ -386,6 +428,48 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL>> {
           acc = acc + p * sh->lv[c * 4 + j];
         }
         if (t < n_t) sT[slot * RS + r * 4 + i] = (Real)acc;
+      }
+    }
+  }
+
+  // Price-CAT tables: the same sums for every category slice, staged through
+  // LDS (more entries than lanes): exponentials, then P(prox), P(dist) and
+  // P(pend) entries; the one-hot pendant rows are P(pend) columns and the
+  // ambiguity rows are formed from them.
+  __device__ __forceinline__ void cat_tables(bool new_pend, bool new_dist) {
+    const int nd = new_dist ? 2 * n_sl * 4 : 0, np = new_pend ? n_sl * 4 : 0;
+    // sE[0 .. 2 n_sl 4): prox then dist; sE[2 n_sl 4 ..): pend
+    for (int t = lane; t < nd + np; t += 64) {
+      const bool pend = t >= nd;
+      const int u = pend ? t - nd : t;
+      const int which = pend ? 2 : u / (n_sl * 4), rem = u % (n_sl * 4), s = rem / 4, k = rem % 4;
+      const double bl = which == 0 ? prox_bl : which == 1 ? dist_bl : pend_bl;
+      sE[which * n_sl * 4 + rem] = exp((sh->rates[s] * bl) * sh->l[k]);
+    }
+    wave_lds_sync();
+    const int ndp = new_dist ? 2 * n_sl * 16 : 0, npp = new_pend ? n_sl * 16 : 0;
+    for (int t = lane; t < ndp + npp; t += 64) {
+      const bool pend = t >= ndp;
+      const int u = pend ? t - ndp : t;
+      const int which = pend ? 2 : u / (n_sl * 16), rem = u % (n_sl * 16);
+      const int s = rem / 16, i = (rem / 4) % 4, j = rem % 4;
+      const double *ex = sE + (which * n_sl + s) * 4;
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc = acc + (ex[k] * sh->x[i * 4 + k]) * sh->xit[j * 4 + k];
+      if (which == 2) sT[(s * kMaxCodes + j) * 4 + i] = (Real)acc;  // one-hot row j, entry i
+      else (which ? sPd : sPp)[rem] = (Real)acc;
+    }
+    if (new_pend && sh->n_slots > 4) {
+      wave_lds_sync();
+      const int n_amb = sh->n_slots - 4;
+      for (int t = lane; t < n_sl * n_amb * 4; t += 64) {
+        const int s = t / (n_amb * 4), slot = 4 + (t / 4) % n_amb, i = t % 4;
+        const int c = sh->slot_code[slot];
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc = acc + (double)sT[(s * kMaxCodes + j) * 4 + i] * sh->lv[c * 4 + j];
+        sT[(s * kMaxCodes + slot) * 4 + i] = (Real)acc;
       }
     }
   }

@@ -71,10 +71,12 @@ __device__ __forceinline__ void prot_copy_model(ProtShared *sh, const DevModel *
 // ---------------------------------------------------------------------------
 // worker waves
 // ---------------------------------------------------------------------------
+// site_cat: Price-CAT (R == 1), the exponential tables of site s are those of
+// its category site_cat[s]; nullptr: the tables of rate r.
 template <typename Real, int R>
 __device__ __forceinline__ void prot_worker(char *smem, Real *scr, int cap, int N, const uint8_t *codes,
                                             const Real *darr, const Real *parr, const int *darr_e,
-                                            const int *parr_e) {
+                                            const int *parr_e, const int *site_cat) {
   constexpr int S = kProtS, RS = R * S;
   using Ops = RealOps<Real>;
   ProtShared *sh = reinterpret_cast<ProtShared *>(smem);
@@ -139,13 +141,14 @@ __device__ __forceinline__ void prot_worker(char *smem, Real *scr, int cap, int 
           const bool act = p < n_pairs;
           const int pc = act ? p : 0;
           const int m = pc / R, r = pc % R;
+          const int er = site_cat ? site_cat[sidx[m]] : r;  // exponential table
           Real util;
           if (nd) {
             Real bd[S], bp[S], h[S];
 #pragma unroll
             for (int kk = 0; kk < S; kk++) {
-              bd[kk] = (Real)sh->ed[r * S + kk] * Ad[(size_t)kk * cap + pc];
-              bp[kk] = (Real)sh->ep[r * S + kk] * Ap[(size_t)kk * cap + pc];
+              bd[kk] = (Real)sh->ed[er * S + kk] * Ad[(size_t)kk * cap + pc];
+              bp[kk] = (Real)sh->ep[er * S + kk] * Ap[(size_t)kk * cap + pc];
             }
 #pragma unroll 1
             for (int i = 0; i < S; i++) {
@@ -166,12 +169,12 @@ __device__ __forceinline__ void prot_worker(char *smem, Real *scr, int cap, int 
             for (int kk = 0; kk < S; kk++) {
               const Real g = (Real)sh->lt[c * S + kk] * h[kk];
               if (act) Gs[(size_t)kk * cap + pc] = g;
-              util = kk == 0 ? (Real)sh->en[r * S] * g : Ops::madd(util, (Real)sh->en[r * S + kk], g);
+              util = kk == 0 ? (Real)sh->en[er * S] * g : Ops::madd(util, (Real)sh->en[er * S + kk], g);
             }
           } else {
-            util = (Real)sh->en[r * S] * Gs[pc];
+            util = (Real)sh->en[er * S] * Gs[pc];
 #pragma unroll
-            for (int kk = 1; kk < S; kk++) util = Ops::madd(util, (Real)sh->en[r * S + kk], Gs[(size_t)kk * cap + pc]);
+            for (int kk = 1; kk < S; kk++) util = Ops::madd(util, (Real)sh->en[er * S + kk], Gs[(size_t)kk * cap + pc]);
           }
           double u = act ? (double)util : 0.0;
           if (R == 4) {
@@ -212,10 +215,11 @@ struct ProtMaster : ThreeTaxControl<ProtMaster<Real, R>> {
   static constexpr int S = kProtS;
   ProtShared *sh;
   int *sidx;
-  int lane, n_workers, cap_sites, n_inf;
+  int lane, n_workers, cap_sites, n_inf, n_sl;
   double log_R;
 
   __device__ __forceinline__ void init(char *smem, const DevModel *dm, int cap_pairs) {
+    n_sl = dm->n_sl;
     sh = reinterpret_cast<ProtShared *>(smem);
     sidx = ProtLayout::sidx(smem);
     lane = threadIdx.x & 63;
@@ -276,15 +280,16 @@ struct ProtMaster : ThreeTaxControl<ProtMaster<Real, R>> {
 
   // The exponentials of Diagd.multi_exp for the changed branch lengths.
   __device__ __forceinline__ void tables(bool new_pend, bool new_dist) {
+    // one table per transition-matrix slice: rate (gamma) or category (CAT)
     if (new_dist) {
-      for (int t = lane; t < 2 * R * S; t += 64) {
-        const int which = t / (R * S), rem = t % (R * S), r = rem / S, k = rem % S;
+      for (int t = lane; t < 2 * n_sl * S; t += 64) {
+        const int which = t / (n_sl * S), rem = t % (n_sl * S), r = rem / S, k = rem % S;
         const double e = exp((sh->rates[r] * (which ? dist_bl : prox_bl)) * sh->l[k]);
         (which ? sh->ed : sh->ep)[rem] = e;
       }
     }
     if (new_pend) {
-      for (int t = lane; t < R * S; t += 64) {
+      for (int t = lane; t < n_sl * S; t += 64) {
         const int r = t / S, k = t % S;
         sh->en[t] = exp((sh->rates[r] * pend_bl) * sh->l[k]);
       }

@@ -141,11 +141,13 @@ static void dediag(double *dst, const double *u, const double *lambda, const dou
     }
 }
 
-void multi_exp(const HostModel &m, double bl, double *dst) {  // diagd.ml:79-100
+// diagd.ml:79-100; with Price-CAT one matrix per category (gcat_model.ml:70-82)
+void multi_exp(const HostModel &m, double bl, double *dst) {
   int S = m.S;
   double util[kMaxStates];
-  for (int r = 0; r < m.R; r++) {
-    for (int i = 0; i < S; i++) util[i] = std::exp(m.rates[r] * bl * m.l[i]);
+  const double *rates = m.sl_rates();
+  for (int r = 0; r < m.n_sl(); r++) {
+    for (int i = 0; i < S; i++) util[i] = std::exp(rates[r] * bl * m.l[i]);
     dediag(dst + r * S * S, m.x.data(), util, m.xit.data(), S);
   }
 }
@@ -202,9 +204,12 @@ static void fill_codes(HostModel &m) {
 int build_model(HostModel &m, int S, int R, const double *exch, const double *statd,
                 const double *rates, std::string &err) {
   if (!(S == 4 || S == 20)) { err = "n_states must be 4 or 20"; return PP_EINVAL; }
-  if (R < 1 || R > kMaxRates) { err = "n_rates must be in 1..8"; return PP_EINVAL; }
+  if (R < 1 || R > kMaxRates) { err = "n_rates must be in 1..32"; return PP_EINVAL; }
   m.S = S;
   m.R = R;
+  m.n_cats = 0;
+  m.cat_rates.clear();
+  m.site_cat.clear();
   m.statd.assign(statd, statd + S);
   m.rates.assign(rates, rates + R);
   for (int i = 0; i < S; i++)
@@ -251,9 +256,11 @@ void HostModel::fill_dev(DevModel *d) const {
   d->n_codes = n_codes;
   d->n_inf = n_inf;
   d->log_R = std::log((double)(float)R);
+  d->n_sl = n_sl();
+  d->cat = n_cats > 0;
   for (int i = 0; i < S; i++) { d->statd[i] = statd[i]; d->l[i] = l[i]; }
   for (int i = 0; i < S * S; i++) { d->x[i] = x[i]; d->xit[i] = xit[i]; }
-  for (int r = 0; r < R; r++) d->rates[r] = rates[r];
+  for (int r = 0; r < n_sl(); r++) d->rates[r] = sl_rates()[r];
   for (int i = 0; i < n_codes * S; i++) d->lv[i] = lv[i];
 }
 

@@ -18,7 +18,9 @@ struct Res {  // one optimised (query, edge): Three_tax.get_results + location
 struct CodeLogArgs {
   const double *sn;     // [N][Epad][R*4] FP64 supernodes
   int N, Epad, n_inf;
-  const double *ttab;   // [n_inf][R*4] character LVs evolved by start_pend
+  const double *ttab;   // [n_tsl][n_codes][R*S] character LVs evolved by start_pend
+  int n_codes;
+  const int *site_cat;  // Price-CAT: table of each site (nullptr: table 0)
   double *lt64;         // [N][n_inf+1][Epad], or
   float *lt32;          // the same in FP32 (when non-null)
 };
@@ -56,6 +58,7 @@ struct PitchArgs {
   // 20-state engine: per-workgroup pair scratch (ProtLayout::scratch_reals(cap_pairs) each)
   void *scratch;
   int cap_pairs;
+  const int *site_cat;  // Price-CAT: category of each pre-masked site (else nullptr)
 };
 
 struct KeepArgs {
@@ -90,6 +93,7 @@ struct OptArgs {
   unsigned long long *n_calls;  // [0]: probes; [2], [3]: as in PitchArgs
   void *scratch;                // 20-state engine, as in PitchArgs
   int cap_pairs;
+  const int *site_cat;          // as in PitchArgs
 };
 
 struct FinalArgs {

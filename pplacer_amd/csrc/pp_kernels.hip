@@ -25,10 +25,11 @@ namespace pp {
 template <int RS>
 __global__ __launch_bounds__(256) void code_log_kernel(CodeLogArgs a) {
   __shared__ double tt[kMaxCodes * RS];
-  for (int t = threadIdx.x; t < a.n_inf * RS; t += blockDim.x) tt[t] = a.ttab[t];
+  const int site = blockIdx.y;
+  const double *tab = a.ttab + (a.site_cat ? (size_t)a.site_cat[site] * a.n_codes * RS : 0);
+  for (int t = threadIdx.x; t < a.n_inf * RS; t += blockDim.x) tt[t] = tab[t];
   __syncthreads();
   const int e = blockIdx.x * blockDim.x + threadIdx.x;  // < Epad by grid construction
-  const int site = blockIdx.y;
   double s[RS];
   const double *sp = a.sn + ((size_t)site * a.Epad + e) * RS;
 #pragma unroll
@@ -236,23 +237,23 @@ __device__ __forceinline__ void pitch_loop(const PitchArgs &a, Master &m, uint8_
   m.command(CMD_EXIT);
 }
 
-template <typename Real, int R, int MAXBD, int SPL>
+template <typename Real, int R, int MAXBD, int SPL, bool CAT>
 __global__ __launch_bounds__(MAXBD) void pitch_mw_kernel(PitchArgs a) {
   extern __shared__ __align__(16) char smem[];
-  using L = MWLayout<Real, R>;
+  const MWLayout<Real, R> lay = mw_layout<Real, R>(a.model);
   MWShared *sh = reinterpret_cast<MWShared *>(smem);
   mw_copy_model(sh, a.model);
   __syncthreads();
   const Real *darr = reinterpret_cast<const Real *>(a.darr);
   const Real *parr = reinterpret_cast<const Real *>(a.parr);
   // taken[e]: read and written only by the lane that owns edge e (e % 64)
-  uint8_t *taken = reinterpret_cast<uint8_t *>(smem + L::bytes(a.N));
+  uint8_t *taken = reinterpret_cast<uint8_t *>(smem + lay.bytes(a.N));
   if (wave_id() != 0) {
-    char *wl = smem + L::align(L::bytes(a.N) + a.E);
-    mw_worker<Real, R, SPL>(smem, wl, a.N, a.codes, darr, parr, a.darr_e, a.parr_e);
+    char *wl = smem + lay.align(lay.bytes(a.N) + a.E);
+    mw_worker<Real, R, SPL, CAT>(smem, lay, wl, a.N, a.codes, darr, parr, a.darr_e, a.parr_e, a.site_cat);
     return;
   }
-  MWMaster<Real, R, SPL> m;
+  MWMaster<Real, R, SPL, CAT> m;
   m.init(smem, a.model);
   pitch_loop(a, m, taken);
 }
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(512) void pitch_prot_kernel(PitchArgs a) {
   if (wave_id() != 0) {
     Real *scr = reinterpret_cast<Real *>(a.scratch) + ProtLayout::scratch_reals(a.cap_pairs) * blockIdx.x;
     prot_worker<Real, R>(smem, scr, a.cap_pairs, a.N, a.codes, reinterpret_cast<const Real *>(a.darr),
-                         reinterpret_cast<const Real *>(a.parr), a.darr_e, a.parr_e);
+                         reinterpret_cast<const Real *>(a.parr), a.darr_e, a.parr_e, a.site_cat);
     return;
   }
   ProtMaster<Real, R> m;
@@ -339,21 +340,21 @@ __device__ __forceinline__ void optimize_loop(const OptArgs &a, Master &m) {
   m.command(CMD_EXIT);
 }
 
-template <typename Real, int R, int MAXBD, int SPL>
+template <typename Real, int R, int MAXBD, int SPL, bool CAT>
 __global__ __launch_bounds__(MAXBD) void optimize_mw_kernel(OptArgs a) {
-  using L = MWLayout<Real, R>;
   extern __shared__ __align__(16) char smem[];
+  const MWLayout<Real, R> lay = mw_layout<Real, R>(a.model);
   MWShared *sh = reinterpret_cast<MWShared *>(smem);
   mw_copy_model(sh, a.model);
   __syncthreads();
   const Real *darr = reinterpret_cast<const Real *>(a.darr);
   const Real *parr = reinterpret_cast<const Real *>(a.parr);
   if (wave_id() != 0) {
-    char *wl = smem + L::align(L::bytes(a.N));
-    mw_worker<Real, R, SPL>(smem, wl, a.N, a.codes, darr, parr, a.darr_e, a.parr_e);
+    char *wl = smem + lay.align(lay.bytes(a.N));
+    mw_worker<Real, R, SPL, CAT>(smem, lay, wl, a.N, a.codes, darr, parr, a.darr_e, a.parr_e, a.site_cat);
     return;
   }
-  MWMaster<Real, R, SPL> m;
+  MWMaster<Real, R, SPL, CAT> m;
   m.init(smem, a.model);
   optimize_loop(a, m);
 }
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(512) void optimize_prot_kernel(OptArgs a) {
   if (wave_id() != 0) {
     Real *scr = reinterpret_cast<Real *>(a.scratch) + ProtLayout::scratch_reals(a.cap_pairs) * blockIdx.x;
     prot_worker<Real, R>(smem, scr, a.cap_pairs, a.N, a.codes, reinterpret_cast<const Real *>(a.darr),
-                         reinterpret_cast<const Real *>(a.parr), a.darr_e, a.parr_e);
+                         reinterpret_cast<const Real *>(a.parr), a.darr_e, a.parr_e, a.site_cat);
     return;
   }
   ProtMaster<Real, R> m;
@@ -597,7 +598,9 @@ struct TreeArgs {
   const int *level_nodes;   // node ids of this launch
   int n_level;
   const int *child_start, *children, *parent;
-  const double *P, *Phalf;  // [G][R][S][S]
+  const double *P, *Phalf;  // [G][n_sl][S][S]
+  const int *site_cat;      // Price-CAT: slice of each site (nullptr: slice = rate)
+  int n_sl;
   const uint8_t *leaf_codes;  // [G][N]
   const double *lv;           // [n_codes][S]
   double *darr, *parr, *evd;  // [G][N][RS]
@@ -640,7 +643,8 @@ __global__ void distal_level_kernel(TreeArgs a) {
   for (int k = 0; k < S; k++) a.darr[row * RS + r * S + k] = v[k];
   if (r == 0) a.darr_e[row] = e;
   double w[S];
-  evolve_rate<S>(a.P + ((size_t)node * R + r) * S * S, v, w);
+  const int sl = a.site_cat ? a.site_cat[site] : r;
+  evolve_rate<S>(a.P + ((size_t)node * a.n_sl + sl) * S * S, v, w);
 #pragma unroll
   for (int k = 0; k < S; k++) a.evd[row * RS + r * S + k] = w[k];
   if (r == 0) a.evd_e[row] = e;
@@ -663,7 +667,8 @@ __global__ void proximal_level_kernel(TreeArgs a) {
 #pragma unroll
   for (int k = 0; k < S; k++) src[k] = a.parr[prow * RS + r * S + k];
   int e = a.parr_e[prow];
-  evolve_rate<S>(a.P + ((size_t)par * R + r) * S * S, src, v);
+  const int sl = a.site_cat ? a.site_cat[site] : r;
+  evolve_rate<S>(a.P + ((size_t)par * a.n_sl + sl) * S * S, src, v);
   for (int c = a.child_start[par]; c < a.child_start[par + 1]; c++) {
     const int sib = a.children[c];
     if (sib == child) continue;
@@ -717,7 +722,8 @@ __global__ void finish_kernel(TreeArgs a, const double *statd, int E, int Epad, 
   }
   if (node < E) {
     double ud[S], up[S];
-    const double *Ph = a.Phalf + ((size_t)node * R + r) * S * S;
+    const int sl = a.site_cat ? a.site_cat[site] : r;
+    const double *Ph = a.Phalf + ((size_t)node * a.n_sl + sl) * S * S;
     evolve_rate<S>(Ph, d, ud);
     evolve_rate<S>(Ph, p, up);
     const size_t srow = (size_t)site * Epad + node;

@@ -114,6 +114,19 @@ class Placer:
         check(lib().pp_model_build(self._ctx, S, R, _p(exch), _p(statd), _p(rates)), "model_build")
         self.n_states, self.n_rates = S, R
 
+    def build_model_cat(self, exch, statd, cat_rates, site_cat):
+        """Price-CAT model (gcat_model.ml:33-95): rate-free GLVs, site i
+        evolving at cat_rates[site_cat[i]].  site_cat indexes the pre-masked
+        sites, 0-based (Gcat_model.mask_sites, gcat_model.ml:429-439)."""
+        exch, statd, cat_rates = _f64(exch), _f64(statd), _f64(cat_rates)
+        site_cat = np.ascontiguousarray(site_cat, dtype=np.int32)
+        S = statd.shape[0]
+        check(lib().pp_model_build_cat(self._ctx, S, cat_rates.shape[0], _p(exch), _p(statd),
+                                       _p(cat_rates)), "model_build_cat")
+        check(lib().pp_site_categories(self._ctx, site_cat.shape[0], _p(site_cat, C.c_int)),
+              "site_categories")
+        self.n_states, self.n_rates = S, 1
+
     def model_diagd(self):
         S = self.n_states
         x, l, xit = np.empty((S, S)), np.empty(S), np.empty((S, S))

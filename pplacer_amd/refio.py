@@ -216,32 +216,43 @@ def write_fasta(path: str, records) -> None:
 
 @dataclass
 class PhyloModel:
-    """refpkg phylo_model contents used by the gamma mixture model."""
+    """refpkg phylo_model contents: the gamma mixture model
+    (gmix_model.ml:386-413) or the Price-CAT model (gcat_model.ml:453-478)."""
 
     subs_model: str
     empirical_frequencies: bool
-    alpha: float
+    alpha: float | None
     n_cats: int
     subs_rates: list | None  # ac ag at cg ct gt for GTR
+    ras_model: str = "gamma"
+    cat_rates: list | None = None  # Price-CAT "Rates"
+    site_categories: list | None = None  # Price-CAT "SiteCategories", shifted to 0-based
 
 
 def parse_phylo_model(obj) -> PhyloModel:
-    """gmix_model.ml:386-413 (init_of_json)."""
+    """The ras_model dispatch of Refpkg.get_model (refpkg.ml:38-63) over
+    Gmix_model.init_of_json (gmix_model.ml:386-413) and
+    Gcat_model.init_of_json (gcat_model.ml:453-478)."""
     if isinstance(obj, str):
         obj = json.loads(obj)
-    if obj.get("ras_model") != "gamma":
-        raise ValueError("Whoops! This is supposed to be a gamma mixture model.")
+    ras = obj.get("ras_model")
+    if ras not in ("gamma", "Price-CAT"):
+        raise ValueError("invalid ras_model: %s" % ras)
     rates = None
     if "subs_rates" in obj:
         sr = obj["subs_rates"]
         rates = [float(sr[k]) for k in ("ac", "ag", "at", "cg", "ct", "gt")]
-    return PhyloModel(
-        subs_model=obj["subs_model"],
-        empirical_frequencies=bool(obj["empirical_frequencies"]),
-        alpha=float(obj["gamma"]["alpha"]),
-        n_cats=int(obj["gamma"]["n_cats"]),
-        subs_rates=rates,
-    )
+    common = dict(subs_model=obj["subs_model"],
+                  empirical_frequencies=bool(obj["empirical_frequencies"]), subs_rates=rates)
+    if ras == "gamma":
+        return PhyloModel(alpha=float(obj["gamma"]["alpha"]), n_cats=int(obj["gamma"]["n_cats"]),
+                          **common)
+    pc = obj["Price-CAT"]
+    cat_rates = [float(x) for x in pc["Rates"]]
+    # FastTree writes 1-based categories (gcat_model.ml:475)
+    cats = [int(x) - 1 for x in pc["SiteCategories"]]
+    return PhyloModel(alpha=None, n_cats=len(cat_rates), ras_model="Price-CAT",
+                      cat_rates=cat_rates, site_categories=cats, **common)
 
 
 def read_phylo_model(path: str) -> PhyloModel:

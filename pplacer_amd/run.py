@@ -1,6 +1,6 @@
 """File-level driver: the Pplacer_run.run_file / run_placements flow
-(pplacer_src/pplacer_run.ml:167-783) for the gamma mixture model, with the
-placement itself on the GPU.
+(pplacer_src/pplacer_run.ml:167-783) for the gamma mixture and Price-CAT
+models, with the placement itself on the GPU.
 
     python -m pplacer_amd.run --tree t.nwk --aln ref_and_queries.fasta \
         --model phylo_model.json -o out.jplace [--fp32]
@@ -8,7 +8,7 @@ placement itself on the GPU.
 Reads the reference tree (edge numbers = pplacer's post-order ids), one
 alignment holding reference and query sequences (a sequence whose name is a
 leaf of the tree is a reference, pplacer_run.ml:658-680), and a refpkg
-phylo_model JSON (gmix_model.ml:386-413).  Protein models need the
+phylo_model JSON (gmix_model.ml:386-413, gcat_model.ml:453-478).  Protein models need the
 exchangeability table passed as JSON (`--exch`, `{"lower": rows, "statd": [...]}`).
 """
 from __future__ import annotations
@@ -32,6 +32,18 @@ def partition(aln, tree: refio.Tree):
     return refs, queries
 
 
+def cat_sites(site_categories, mask):
+    """Gcat_model.mask_sites (gcat_model.ml:429-439): keep the categories of
+    the columns the pre-mask keeps.  The reference leaves them untouched when
+    the lengths differ and its model check then rejects the alignment; here
+    that is an error straight away."""
+    mask = np.asarray(mask, dtype=bool)
+    if len(site_categories) != mask.shape[0]:
+        raise ValueError("Price-CAT model has %d site categories for an alignment of %d columns"
+                         % (len(site_categories), mask.shape[0]))
+    return np.asarray(site_categories, dtype=np.int32)[mask]
+
+
 def run_placements(tree: refio.Tree, refs, queries, pm: refio.PhyloModel, exch_table=None,
                    prefs=None, precision=placer.PP_FP64, device=0):
     """Returns (pqueries, status) where pqueries is [(namlom, [PlacementRecord])]."""
@@ -46,7 +58,6 @@ def run_placements(tree: refio.Tree, refs, queries, pm: refio.PhyloModel, exch_t
         exch = refio.exchangeabilities_from_lower(exch_table["lower"])
         statd = (placer.emper_freq(20, ref_seqs) if pm.empirical_frequencies
                  else np.asarray(exch_table["statd"], dtype=float))
-    rates = placer.discrete_gamma(pm.n_cats, pm.alpha)
     # pre-mask (pplacer_run.ml:59-144) and de-duplication (:228-253)
     mask, keep, n = placer.premask(S, ref_seqs, [s for _, s in queries])
     cut_q = placer.cut_columns([s for _, s in queries], mask)
@@ -55,7 +66,10 @@ def run_placements(tree: refio.Tree, refs, queries, pm: refio.PhyloModel, exch_t
     leaf = {i: name_to_seq[tree.names[i]] for i in tree.leaves()}
     pl = placer.Placer(device)
     try:
-        pl.build_model(exch, statd, rates)
+        if pm.ras_model == "Price-CAT":
+            pl.build_model_cat(exch, statd, pm.cat_rates, cat_sites(pm.site_categories, mask))
+        else:
+            pl.build_model(exch, statd, placer.discrete_gamma(pm.n_cats, pm.alpha))
         pl.set_tree(tree, leaf, n)
         res, status, stats = pl.place([s.upper() for _, s in uniq], prefs=prefs,
                                       precision=precision)

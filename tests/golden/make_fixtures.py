@@ -8,6 +8,9 @@ Run once in a container where /root/reference exists:
   log-likelihoods are recorded in like/expected.json from test_like.ml:105-124).
 * prot_models.json: the WAG, JTT and LG exchangeabilities and frequencies, the
   numeric tables of pplacer_src/prot_models.ml:70-163, as plain numbers.
+* price_cat/ar53_phylo_model.json: the Price-CAT phylo_model of the donovan
+  regression refpkg (donovan-test/ar53_pplacer_testset/gtdb_r226_ar53.refpkg),
+  the same JSON object written without indentation.
 """
 import json
 import os
@@ -62,7 +65,18 @@ def copy_jplace():
             os.chmod(dst, 0o644)
 
 
+def copy_price_cat():
+    src = os.path.join(REF, "donovan-test/ar53_pplacer_testset/gtdb_r226_ar53.refpkg",
+                       "phylo_model59kpwu4z.json")
+    os.makedirs(os.path.join(HERE, "price_cat"), exist_ok=True)
+    with open(src) as f:
+        obj = json.load(f)
+    with open(os.path.join(HERE, "price_cat", "ar53_phylo_model.json"), "w") as f:
+        json.dump(obj, f, separators=(",", ":"))
+
+
 if __name__ == "__main__":
     copy_like()
     prot_tables()
     copy_jplace()
+    copy_price_cat()

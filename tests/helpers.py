@@ -34,6 +34,41 @@ def oracle_setup(po, wl, premask=True, **prefs):
     return model, pre, core, queries
 
 
+def oracle_setup_cat(po, wl, cat_rates, site_cat, premask=True, **prefs):
+    """oracle_setup for the Price-CAT model: site_cat gives a category per
+    column of the un-masked alignment and is cut by the pre-mask as
+    Gcat_model.mask_sites does (gcat_model.ml:429-439).  Returns a dict with
+    the oracle objects and the pre-masked leaf rows and categories."""
+    S = wl.n_states
+    ref_seqs = [s for _, s in wl.ref_aln]
+    statd = po.emper_freq(S, ref_seqs)
+    queries = [s for _, s in wl.queries]
+    leaf = wl.leaf_seqs()
+    site_cat = np.asarray(site_cat, dtype=np.int32)
+    assert site_cat.shape[0] == len(ref_seqs[0])
+    if premask:
+        st, mask, keep, n = po.premask(S, ref_seqs, queries)
+        assert st == 0
+        queries = [cut(q, mask) for q in queries]
+        leaf = {i: cut(s, mask) for i, s in leaf.items()}
+        site_cat = site_cat[np.asarray(mask, dtype=bool)]
+    N = len(next(iter(leaf.values())))
+    model = po.Model(wl.exch, statd, cat_rates=cat_rates, site_cat=site_cat)
+    pre = po.Precompute(model, wl.tree, leaf, N)
+    pre.pull_and_supernodes()
+    core = po.Core(pre, **prefs)
+    return dict(model=model, pre=pre, core=core, queries=queries, leaf=leaf, N=N,
+                site_cat=site_cat, statd=statd)
+
+
+def ar53_price_cat():
+    """(cat_rates, site_categories 0-based) of the donovan refpkg's Price-CAT
+    WAG model (20 categories over 8062 columns)."""
+    from pplacer_amd import refio
+    pm = refio.read_phylo_model(os.path.join(GOLDEN, "price_cat", "ar53_phylo_model.json"))
+    return pm.cat_rates, np.asarray(pm.site_categories, dtype=np.int32)
+
+
 def like_fixture(po, name):
     from pplacer_amd import refio
 

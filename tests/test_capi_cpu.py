@@ -60,3 +60,49 @@ def test_prefs_defaults():
     assert (p.start_pend, p.max_pend, p.initial_tolerance, p.strike_box, p.keep_factor) == \
         (0.1, 2.0, 0.01, 3.0, 0.01)
     assert (p.max_strikes, p.max_pitches, p.keep_at_most) == (6, 40, 7)
+
+
+def test_price_cat_phylo_model_fixture():
+    """The donovan refpkg's Price-CAT phylo_model (refpkg.ml:38-63,
+    gcat_model.ml:453-478): 20 FastTree rates, 8062 1-based categories
+    shifted to 0-based."""
+    from pplacer_amd import refio
+    from tests.helpers import ar53_price_cat
+    pm = refio.read_phylo_model(os.path.join(GOLDEN, "price_cat", "ar53_phylo_model.json"))
+    assert pm.ras_model == "Price-CAT" and pm.subs_model == "WAG"
+    assert not pm.empirical_frequencies
+    assert pm.n_cats == 20 and len(pm.cat_rates) == 20
+    assert pm.cat_rates[0] == 0.046161 and pm.cat_rates[-1] == 18.464318
+    rates, cats = ar53_price_cat()
+    assert cats.shape == (8062,) and cats.min() == 0 and cats.max() == 16
+    assert cats[:4].tolist() == [10, 8, 8, 7]  # the file's 11, 9, 9, 8
+    with pytest.raises(ValueError, match="invalid ras_model"):
+        refio.parse_phylo_model({"ras_model": "free", "subs_model": "WAG",
+                                 "empirical_frequencies": False})
+
+
+def test_cat_sites_follow_the_premask():
+    """Gcat_model.mask_sites (gcat_model.ml:429-439) keeps the categories of
+    the kept columns; a length mismatch is an error."""
+    from pplacer_amd.run import cat_sites
+    mask = np.array([1, 0, 1, 1, 0], dtype=np.uint8)
+    assert cat_sites([4, 3, 2, 1, 0], mask).tolist() == [4, 2, 1]
+    with pytest.raises(ValueError, match="site categories"):
+        cat_sites([1, 2, 3], mask)
+
+
+def test_cat_model_entry_points_reject_bad_input():
+    """pp_model_build_cat / pp_site_categories argument checks run on the
+    host before any device work."""
+    from pplacer_amd import _lib
+    L = _lib.lib()
+    S = 4
+    exch = np.ones((S, S))
+    statd = np.full(S, 0.25)
+    rates = np.array([0.5, -1.0])
+    dp = C.POINTER(C.c_double)
+    rc = L.pp_model_build_cat(None, S, 2, exch.ctypes.data_as(dp), statd.ctypes.data_as(dp),
+                              rates.ctypes.data_as(dp))
+    assert rc != 0
+    cats = np.zeros(3, dtype=np.int32)
+    assert L.pp_site_categories(None, 3, cats.ctypes.data_as(C.POINTER(C.c_int))) != 0
