@@ -49,8 +49,13 @@ enum {
   PP_Q_FIND_START = 4,     /* Minimization.FindStartFailure */
   PP_Q_INVALID_START = 5,  /* Minimization.InvalidStartValues */
   PP_Q_ASSERT = 6,         /* assert(prox_bl >= 0.) three_tax.ml:43 */
-  PP_Q_RANK_EXHAUSTED = 7  /* internal: ranking window too short (host retries) */
+  PP_Q_RANK_EXHAUSTED = 7, /* internal: ranking window too short (host retries) */
+  PP_Q_MAXITER = 8,        /* Minimization.ExceededMaxIter outside the optimiser (-p) */
+  PP_Q_GSL = 9             /* an uncaught Gsl_exn (-p: EBADFUNC, EBADTOL) */
 };
+
+/* Pendant-length priors for -p (Core.prior, core.ml:54-56) */
+enum { PP_PRIOR_FLAT_EXP = 0, PP_PRIOR_UNIFORM = 1, PP_PRIOR_INFORMATIVE = 2 };
 
 enum { PP_FP64 = 0, PP_FP32 = 1 };
 
@@ -160,6 +165,11 @@ typedef struct {
   int max_strikes;          /* 6 (0 = evaluate every edge) */
   int max_pitches;          /* 40 */
   int keep_at_most;         /* 7 */
+  /* -p (prefs.ml:198-203): marginal likelihoods and posterior probabilities
+   * of the kept placements.  -p also sets keep_at_most 20, keep_factor 0.001
+   * and max_strikes 20; the caller sets those fields as the CLI does. */
+  int calc_pp;              /* 0 */
+  double pp_rel_err;        /* 0.01 (--pp-rel-err) */
 } pp_prefs;
 
 void pp_prefs_default(pp_prefs *p);
@@ -171,6 +181,8 @@ typedef struct {
   double log_like;  /* likelihood */
   double dist_bl;   /* distal_length */
   double pend_bl;   /* pendant_length */
+  double post_prob;      /* post_prob with calc_pp, else NaN (placement.ml:62-65) */
+  double marginal_prob;  /* marginal_like with calc_pp, else NaN */
 } pp_placement;
 
 typedef struct {
@@ -192,6 +204,7 @@ typedef struct {
   int64_t n_site_probes_pend_pitch; /* n_site_probes_pend of the baseball kernels alone */
   int64_t n_site_probes_dist_pitch; /* n_site_probes_dist of the baseball kernels alone */
   int64_t n_pitch_launches;         /* launches of the baseball kernel */
+  double t_pp_ms;                   /* -p marginal likelihoods ("PP calculation", core.ml:380) */
 } pp_stats;
 
 /* Stage 1 for a batch: scores[q][e] = Glv.masked_logdot of the query (full
@@ -214,9 +227,34 @@ int pp_optimize_batch(pp_ctx *ctx, int n_jobs, const char *queries, const int *q
                       double tolerance, double max_pend, int precision, double *ll,
                       double *pend, double *dist, int *n_calls, int *status);
 
+/* The prior Core.pplacer_core is given (pplacer_run.ml:380-389): kind
+ * PP_PRIOR_UNIFORM (means unused), PP_PRIOR_FLAT_EXP (means[0] = the mean),
+ * PP_PRIOR_INFORMATIVE (means[n_nodes] = the mean of each edge id, from
+ * pp_prior_mean_map).  pp_tree_set installs the reference's default, the flat
+ * exponential with mean pp_tree_length / (n_nodes - 1). */
+int pp_set_prior(pp_ctx *ctx, int kind, int n, const double *means);
+
+/* Gtree.tree_length (gtree.ml:102-114) and Core.prior_mean_map with
+ * top_segment_f = (+ lower) (core.ml:34-52) on a tree given as in
+ * pp_tree_set; host-only.  means[n_nodes] (the root entry is unused). */
+int pp_tree_length(int n_nodes, const int *child_start, const int *children, const double *bl,
+                   double *length);
+int pp_prior_mean_map(int n_nodes, const int *child_start, const int *children,
+                      const double *bl, double lower, double *means);
+
+/* Three_tax.calc_marg_prob (three_tax.ml:169-215) per job: the marginal
+ * log-likelihood of the query at (loc, pend, distal) under the context's
+ * prior, with relative tolerance rel_err and the pendant range capped at
+ * max_pend; status is 0 or a Minimization / GSL failure code. */
+int pp_marginal_batch(pp_ctx *ctx, int n_jobs, const char *queries, const int *query_of_job,
+                      const int *locs, const double *pend, const double *dist,
+                      double rel_err, double max_pend, double *marg, int *n_calls,
+                      int *status);
+
 /* Full Core.process_query for a batch of queries: stage 1, ranking,
  * baseball with Brent at initial_tolerance, keep, refine at 1e-5,
- * ll_normalized_prob and the ml_ratio sort.  out is [n_queries][keep_at_most],
+ * ll_normalized_prob, with calc_pp the marginal likelihoods and posterior
+ * probabilities (core.ml:378-403), and the ml_ratio sort.  out is [n_queries][keep_at_most],
  * n_out[q] placements each; status[q] is a PP_Q_* code. */
 int pp_place_batch(pp_ctx *ctx, int n_queries, const char *queries, const pp_prefs *prefs,
                    int precision, pp_placement *out, int *n_out, int *status,

@@ -878,6 +878,127 @@ int or_brent(or_fun f, void *ctx, double raw_start, double left, double right,
 }
 
 /* ======================================================================== */
+/* integration.ml: GSL 2.x gsl_integration_qng = QUADPACK QNG                */
+/* ======================================================================== */
+
+#define PP_QNG_STORAGE static const
+#include "../pplacer_amd/csrc/pp_qng_tables.h"
+
+/* QUADPACK's error estimate from the difference of two rules, the integral
+ * of |f| and of |f - mean| (rescale_error of QUADPACK/GSL). */
+static double qng_rescale_error(double err, double result_abs, double result_asc)
+{
+  err = fabs(err);
+  if (result_asc != 0 && err != 0) {
+    double scale = pow(200 * err / result_asc, 1.5);
+    err = scale < 1 ? result_asc * scale : result_asc;
+  }
+  if (result_abs > DBL_MIN / (50 * DBL_EPSILON)) {
+    double min_err = 50 * DBL_EPSILON * result_abs;
+    if (min_err > err) err = min_err;
+  }
+  return err;
+}
+
+#define QNG_F(x) (f((x), ctx))
+#define QNG_STOP() do { if (stop && *stop) return OR_QNG_ABORT; } while (0)
+
+/* QUADPACK QNG (Piessens et al. 1983): the 21-point Gauss-Kronrod rule with
+ * the 10-point Gauss error estimate, then Patterson's 43- and 87-point
+ * extensions, reusing every earlier evaluation; each stage returns when its
+ * rescaled error is below epsabs or epsrel*|result|.  f is evaluated at the
+ * centre, then at +/- pairs in table order (plus first). */
+int or_qng(or_fun f, void *ctx, double a, double b, double epsabs, double epsrel,
+           double *result, double *abserr, int *neval, const int *stop)
+{
+  double fv1[5], fv2[5], fv3[5], fv4[5], savfun[21];
+  const double half_length = 0.5 * (b - a), abs_half_length = fabs(half_length);
+  const double center = 0.5 * (b + a);
+  *result = 0; *abserr = 0; *neval = 0;
+  if (epsabs <= 0 && (epsrel < 50 * DBL_EPSILON || epsrel < 0.5e-28)) return OR_QNG_EBADTOL;
+  const double f_center = QNG_F(center);
+  QNG_STOP();
+  double res10 = 0, res21 = pp_qng_w21b[5] * f_center;
+  double resabs = pp_qng_w21b[5] * fabs(f_center);
+  for (int k = 0; k < 5; k++) {
+    const double abscissa = half_length * pp_qng_x1[k];
+    const double fval1 = QNG_F(center + abscissa);
+    QNG_STOP();
+    const double fval2 = QNG_F(center - abscissa);
+    QNG_STOP();
+    const double fval = fval1 + fval2;
+    res10 += pp_qng_w10[k] * fval;
+    res21 += pp_qng_w21a[k] * fval;
+    resabs += pp_qng_w21a[k] * (fabs(fval1) + fabs(fval2));
+    savfun[k] = fval; fv1[k] = fval1; fv2[k] = fval2;
+  }
+  for (int k = 0; k < 5; k++) {
+    const double abscissa = half_length * pp_qng_x2[k];
+    const double fval1 = QNG_F(center + abscissa);
+    QNG_STOP();
+    const double fval2 = QNG_F(center - abscissa);
+    QNG_STOP();
+    const double fval = fval1 + fval2;
+    res21 += pp_qng_w21b[k] * fval;
+    resabs += pp_qng_w21b[k] * (fabs(fval1) + fabs(fval2));
+    savfun[k + 5] = fval; fv3[k] = fval1; fv4[k] = fval2;
+  }
+  resabs *= abs_half_length;
+  const double mean = 0.5 * res21;
+  double resasc = pp_qng_w21b[5] * fabs(f_center - mean);
+  for (int k = 0; k < 5; k++)
+    resasc += (pp_qng_w21a[k] * (fabs(fv1[k] - mean) + fabs(fv2[k] - mean))
+               + pp_qng_w21b[k] * (fabs(fv3[k] - mean) + fabs(fv4[k] - mean)));
+  resasc *= abs_half_length;
+  double result_kronrod = res21 * half_length;
+  double err = qng_rescale_error((res21 - res10) * half_length, resabs, resasc);
+  if (err < epsabs || err < epsrel * fabs(result_kronrod)) {
+    *result = result_kronrod; *abserr = err; *neval = 21;
+    return OR_QNG_OK;
+  }
+  double res43 = pp_qng_w43b[11] * f_center;
+  for (int k = 0; k < 10; k++) res43 += savfun[k] * pp_qng_w43a[k];
+  for (int k = 0; k < 11; k++) {
+    const double abscissa = half_length * pp_qng_x3[k];
+    const double fp = QNG_F(center + abscissa);
+    QNG_STOP();
+    const double fm = QNG_F(center - abscissa);
+    QNG_STOP();
+    const double fval = fp + fm;
+    res43 += fval * pp_qng_w43b[k];
+    savfun[k + 10] = fval;
+  }
+  result_kronrod = res43 * half_length;
+  err = qng_rescale_error((res43 - res21) * half_length, resabs, resasc);
+  if (err < epsabs || err < epsrel * fabs(result_kronrod)) {
+    *result = result_kronrod; *abserr = err; *neval = 43;
+    return OR_QNG_OK;
+  }
+  double res87 = pp_qng_w87b[22] * f_center;
+  for (int k = 0; k < 21; k++) res87 += savfun[k] * pp_qng_w87a[k];
+  for (int k = 0; k < 22; k++) {
+    const double abscissa = half_length * pp_qng_x4[k];
+    const double fp = QNG_F(center + abscissa);
+    QNG_STOP();
+    const double fm = QNG_F(center - abscissa);
+    QNG_STOP();
+    res87 += pp_qng_w87b[k] * (fp + fm);
+  }
+  result_kronrod = res87 * half_length;
+  err = qng_rescale_error((res87 - res43) * half_length, resabs, resasc);
+  *result = result_kronrod; *abserr = err; *neval = 87;
+  if (err < epsabs || err < epsrel * fabs(result_kronrod)) return OR_QNG_OK;
+  return OR_QNG_ETOL;   /* "failed to reach tolerance with highest-order rule" */
+}
+
+/* gsl_ran_exponential_pdf: the density (1/mu) exp(-x/mu) on x >= 0 */
+double or_exponential_pdf(double x, double mu)
+{
+  if (x < 0) return 0;
+  return exp(-x / mu) / mu;
+}
+
+/* ======================================================================== */
 /* three_tax.ml + core.ml                                                    */
 /* ======================================================================== */
 
@@ -960,6 +1081,137 @@ static int tt_optimize(three_tax *tt, double tolerance, double max_pend_bl, int 
     if (fabs(prev_pend - curr_pend) > tolerance || fabs(prev_dist - curr_dist) > tolerance) {
       prev_pend = curr_pend; prev_dist = curr_dist;
     } else return OR_OK;
+  }
+}
+
+/* ---- posterior probabilities (three_tax.ml:109-215, core.ml:86-92) ------- */
+
+typedef struct { int kind; double mu; } prior_fn;
+
+/* core.ml:86-92: the prior of a pendant length at one location */
+static prior_fn prior_at(const or_prefs *pr, int loc)
+{
+  prior_fn p;
+  p.kind = pr->prior_kind;
+  p.mu = pr->prior_kind == OR_PRIOR_INFORMATIVE ? pr->prior_means[loc] : pr->prior_mean;
+  return p;
+}
+static double prior_eval(prior_fn p, double x)
+{
+  return p.kind == OR_PRIOR_UNIFORM ? 1. : or_exponential_pdf(x, p.mu);
+}
+
+/* three_tax.ml:38-39, 41-45 as plain setters */
+static void tt_set_pend(three_tax *tt, double v)
+{
+  tt_set_edge_bl(tt, tt->pend_o, tt->pend_v, &tt->pend_bl, v);
+}
+static int tt_set_dist(three_tax *tt, double v)
+{
+  double prox_bl = (tt->dist_bl + tt->prox_bl) - v;
+  if (!(prox_bl >= 0.)) return OR_BAD_INPUT;   /* assert(prox_bl >= 0.) */
+  tt_set_edge_bl(tt, tt->dist_o, tt->dist_v, &tt->dist_bl, v);
+  tt_set_edge_bl(tt, tt->prox_o, tt->prox_v, &tt->prox_bl, prox_bl);
+  return OR_OK;
+}
+
+typedef struct { three_tax *tt; prior_fn prior; double target; } upper_ctx;
+
+/* find_upper_limit's objective (three_tax.ml:142-149) */
+static double upper_f(double pend_bl, void *c)
+{
+  upper_ctx *u = (upper_ctx *)c;
+  u->tt->n_like_calls++;
+  tt_set_pend(u->tt, pend_bl);
+  double like = tt_log_like(u->tt) + log(prior_eval(u->prior, pend_bl));
+  double x = fabs(u->target - like);
+  return x == INFINITY ? 100000.0 : x;
+}
+
+/* three_tax.ml:121-163 */
+static int find_upper_limit(three_tax *tt, double max_pend, prior_fn prior, double orig_ll,
+                            double *limit)
+{
+  double orig_pend_bl = tt->pend_bl;
+  if (prior_eval(prior, orig_pend_bl) == 0.) {
+    double curr = orig_pend_bl;
+    for (;;) {
+      double next = curr / 2.;
+      if (prior_eval(prior, next) > exp(-10.)) { *limit = curr; return OR_OK; }
+      curr = next;
+    }
+  }
+  upper_ctx u = { tt, prior, -10. + orig_ll + log(prior_eval(prior, orig_pend_bl)) };
+  double start = orig_pend_bl + (max_pend - orig_pend_bl) / 2.;
+  int st = or_brent(upper_f, &u, start, orig_pend_bl, max_pend, 0.02, 10, limit);
+  if (st == OR_GSL_EINVAL) { *limit = max_pend; return OR_OK; }
+  return st;
+}
+
+typedef struct {
+  three_tax *tt;
+  prior_fn prior;
+  double base_ll, upper, rel_err;
+  int stop;   /* an inner integration failed, or an assertion: abandon */
+  int err;    /* OR_QNG_* of the failed inner integration, or OR_BAD_INPUT */
+} marg_ctx;
+
+/* the inner integrand, three_tax.ml:183-186 */
+static double marg_inner_f(double pend_bl, void *c)
+{
+  marg_ctx *m = (marg_ctx *)c;
+  m->tt->n_like_calls++;
+  tt_set_pend(m->tt, pend_bl);
+  return exp(tt_log_like(m->tt) - m->base_ll) * prior_eval(m->prior, pend_bl);
+}
+
+static double marg_inner(marg_ctx *m)
+{
+  double r, e;
+  int n;
+  int st = or_qng(marg_inner_f, m, 0., m->upper, 0., m->rel_err, &r, &e, &n, NULL);
+  if (st != OR_QNG_OK) { m->stop = 1; m->err = st; return 0.; }
+  return r;
+}
+
+/* the outer integrand, three_tax.ml:190-192 */
+static double marg_outer_f(double dist_bl, void *c)
+{
+  marg_ctx *m = (marg_ctx *)c;
+  if (tt_set_dist(m->tt, dist_bl) != OR_OK) { m->stop = 1; m->err = OR_BAD_INPUT; return 0.; }
+  return marg_inner(m);
+}
+
+/* three_tax.ml:169-215 */
+static int calc_marg_prob(three_tax *tt, prior_fn prior, double rel_err, double max_pend,
+                          double *marg)
+{
+  const int max_n_exceptions = 10;
+  tt->n_like_calls++;
+  const double base_ll = tt_log_like(tt);
+  const double cut_bl = tt->dist_bl + tt->prox_bl;
+  double upper;
+  int st = find_upper_limit(tt, max_pend, prior, base_ll, &upper);
+  if (st != OR_OK) return st;
+  for (int n_exceptions = 0;; n_exceptions++) {
+    if (n_exceptions >= max_n_exceptions) { *marg = base_ll; return OR_OK; }
+    marg_ctx m = { tt, prior, base_ll, upper, rel_err, 0, 0 };
+    double v;
+    if (fabs(cut_bl - 0.) < 1e-5) {   /* cut_bl =~ 0. (ppatteries.ml:96) */
+      st = tt_set_dist(tt, cut_bl / 2.);
+      if (st != OR_OK) return st;
+      v = marg_inner(&m);
+    } else {
+      double r, e;
+      int n;
+      int qs = or_qng(marg_outer_f, &m, 0., cut_bl, 0., rel_err, &r, &e, &n, &m.stop);
+      if (!m.stop && qs != OR_QNG_OK) { m.stop = 1; m.err = qs; }
+      v = r / cut_bl;
+    }
+    if (!m.stop) { *marg = base_ll + log(v); return OR_OK; }
+    if (m.err == OR_BAD_INPUT) return OR_BAD_INPUT;
+    if (m.err != OR_QNG_ETOL) return OR_GSL_EINVAL;   /* EBADTOL: uncaught Gsl_exn */
+    upper = upper / 2.;   /* ETOL: halve the pendant range and retry */
   }
 }
 
@@ -1121,6 +1373,39 @@ static int setup_query(query_state *qs, const or_core *c, const char *query, uin
   return OR_OK;
 }
 
+int or_three_tax_log_like(const or_core *c, const char *query, int loc, double pend,
+                          double distal, double *ll)
+{
+  uint8_t *mask = malloc(c->n_sites);
+  query_state qs;
+  int st = setup_query(&qs, c, query, mask);
+  if (st == OR_OK) {
+    set_tt_edges(&qs, loc, pend, distal);
+    *ll = tt_log_like(&qs.tt);
+    free_tt(&qs.tt);
+  }
+  free(mask);
+  return st;
+}
+
+int or_calc_marg_prob(const or_core *c, const char *query, int loc, double pend,
+                      double distal, double *marg, long *n_calls)
+{
+  uint8_t *mask = malloc(c->n_sites);
+  query_state qs;
+  int st = setup_query(&qs, c, query, mask);
+  if (st == OR_OK) {
+    set_tt_edges(&qs, loc, pend, distal);
+    qs.tt.n_like_calls = 0;
+    st = calc_marg_prob(&qs.tt, prior_at(&c->prefs, loc), c->prefs.pp_rel_err,
+                        c->prefs.max_pend, marg);
+    *n_calls = qs.tt.n_like_calls;
+    free_tt(&qs.tt);
+  }
+  free(mask);
+  return st;
+}
+
 int or_three_tax_optimize(const or_core *c, const char *query, int loc, double pend,
                           double distal, double tolerance, double *ll,
                           double *pend_out, double *dist_out, long *n_calls)
@@ -1157,7 +1442,7 @@ static double now_s(void)
 
 typedef struct { int loc; double ll, pend, dist; } ml_result;
 
-/* core.ml:113-408 (no fantasy, no posterior probabilities, Dummy figs) */
+/* core.ml:113-408 (no fantasy, Dummy figs) */
 int or_process_query(const or_core *c, const char *query, or_placement *out,
                      int *n_out, or_stats *stats)
 {
@@ -1239,6 +1524,19 @@ int or_process_query(const or_core *c, const char *query, or_placement *out,
       if (s2 == OR_OK) { kept[i].ll = ll; kept[i].pend = pend; kept[i].dist = dist; }
       else if (!is_gsl(s2)) { st = s2; break; }
     }
+    /* core.ml:378-403: marginal likelihoods and posterior probabilities */
+    double *marg = malloc(sizeof(double) * (n_keep > 0 ? n_keep : 1));
+    double *post = malloc(sizeof(double) * (n_keep > 0 ? n_keep : 1));
+    if (st == OR_OK && pr->calc_pp) {
+      for (int i = 0; i < n_keep && st == OR_OK; i++) {
+        set_tt_edges(&qs, kept[i].loc, kept[i].pend, kept[i].dist);
+        st = calc_marg_prob(&qs.tt, prior_at(pr, kept[i].loc), pr->pp_rel_err, pr->max_pend,
+                            &marg[i]);
+        st_->n_like_calls += qs.tt.n_like_calls;
+        qs.tt.n_like_calls = 0;
+      }
+      if (st == OR_OK) or_ll_normalized_prob(marg, n_keep, post);
+    }
     if (st == OR_OK) {
       double *lls = malloc(sizeof(double) * n_keep), *ratio = malloc(sizeof(double) * n_keep);
       for (int i = 0; i < n_keep; i++) lls[i] = kept[i].ll;
@@ -1251,11 +1549,13 @@ int or_process_query(const or_core *c, const char *query, or_placement *out,
         int j = pk[i].idx;
         out[i].loc = kept[j].loc; out[i].ml_ratio = ratio[j]; out[i].log_like = kept[j].ll;
         out[i].dist_bl = kept[j].dist; out[i].pend_bl = kept[j].pend;
+        out[i].post_prob = pr->calc_pp ? post[j] : NAN;
+        out[i].marginal_prob = pr->calc_pp ? marg[j] : NAN;
       }
       *n_out = n_keep;
       free(lls); free(ratio); free(pk); free(pt);
     }
-    free(kv); free(tmp); free(kept);
+    free(kv); free(tmp); free(kept); free(marg); free(post);
   }
   st_->t_ml += now_s() - t2 + (t2 - t1);
   free(res); free(scores); free(order); free(mask);

@@ -144,10 +144,29 @@ typedef double (*or_fun)(double x, void *ctx);
 int or_brent(or_fun f, void *ctx, double raw_start, double left, double right,
              double tolerance, int max_iters, double *result);
 
+/* ---- Integration (integration.ml:28: GSL 2.x gsl_integration_qng, the
+ * QUADPACK QNG method) ------------------------------------------------------ */
+enum { OR_QNG_OK = 0, OR_QNG_ETOL = 1, OR_QNG_EBADTOL = 2, OR_QNG_ABORT = 3 };
+/* *stop (may be NULL) is read after every evaluation of f; a non-zero value
+ * abandons the rule (an exception raised inside f). */
+int or_qng(or_fun f, void *ctx, double a, double b, double epsabs, double epsrel,
+           double *result, double *abserr, int *neval, const int *stop);
+
+/* ---- pendant-length priors (core.ml:54-56, 86-92; pplacer_run.ml:380-389) */
+enum { OR_PRIOR_FLAT_EXP = 0, OR_PRIOR_UNIFORM = 1, OR_PRIOR_INFORMATIVE = 2 };
+double or_exponential_pdf(double x, double mu);
+
 /* ---- placement core (core.ml, three_tax.ml, fig.ml) --------------------- */
 typedef struct {
   double start_pend, max_pend, initial_tolerance, strike_box, keep_factor;
   int max_strikes, max_pitches, keep_at_most;
+  /* -p (prefs.ml:198-203, 216-223): posterior probabilities of the kept
+   * placements from their marginal likelihoods (Three_tax.calc_marg_prob) */
+  int calc_pp;
+  double pp_rel_err;
+  int prior_kind;              /* OR_PRIOR_* */
+  double prior_mean;           /* flat exponential prior mean */
+  const double *prior_means;   /* informative: mean per edge id */
 } or_prefs;
 
 typedef struct {
@@ -163,6 +182,7 @@ typedef struct {
 typedef struct {
   int loc;
   double ml_ratio, log_like, dist_bl, pend_bl;
+  double post_prob, marginal_prob;  /* with calc_pp (placement.ml:62-65) */
 } or_placement;
 
 typedef struct {
@@ -185,6 +205,13 @@ int or_process_queries(const or_core *c, const char *queries, int n_queries,
                        int n_threads, or_placement *out /* [Q][keep_at_most] */,
                        int *n_out, int *status, or_stats *stats);
 void or_ll_normalized_prob(const double *ll, int n, double *out);
+/* Three_tax.log_like (three_tax.ml:30-36) at (loc, pend, distal) */
+int or_three_tax_log_like(const or_core *c, const char *query, int loc, double pend,
+                          double distal, double *ll);
+/* Three_tax.calc_marg_prob (three_tax.ml:169-215) for the query at loc with
+ * the given branch lengths, with the prior of c->prefs at loc */
+int or_calc_marg_prob(const or_core *c, const char *query, int loc, double pend,
+                      double distal, double *marg, long *n_calls);
 
 #ifdef __cplusplus
 }

@@ -41,7 +41,9 @@ class OrPrefs(C.Structure):
     _fields_ = [("start_pend", C.c_double), ("max_pend", C.c_double),
                 ("initial_tolerance", C.c_double), ("strike_box", C.c_double),
                 ("keep_factor", C.c_double), ("max_strikes", C.c_int),
-                ("max_pitches", C.c_int), ("keep_at_most", C.c_int)]
+                ("max_pitches", C.c_int), ("keep_at_most", C.c_int), ("calc_pp", C.c_int),
+                ("pp_rel_err", C.c_double), ("prior_kind", C.c_int), ("prior_mean", C.c_double),
+                ("prior_means", _dp)]
 
 
 class OrCore(C.Structure):
@@ -52,7 +54,8 @@ class OrCore(C.Structure):
 
 class OrPlacement(C.Structure):
     _fields_ = [("loc", C.c_int), ("ml_ratio", C.c_double), ("log_like", C.c_double),
-                ("dist_bl", C.c_double), ("pend_bl", C.c_double)]
+                ("dist_bl", C.c_double), ("pend_bl", C.c_double), ("post_prob", C.c_double),
+                ("marginal_prob", C.c_double)]
 
 
 class OrStats(C.Structure):
@@ -127,6 +130,11 @@ def lib():
             "or_process_queries": (i, [C.POINTER(OrCore), C.c_char_p, i, i,
                                        C.POINTER(OrPlacement), _ip, _ip, C.POINTER(OrStats)]),
             "or_ll_normalized_prob": (None, [_dp, i, _dp]),
+            "or_qng": (i, [OR_FUN, v, d, d, d, d, _dp, _dp, _ip, _ip]),
+            "or_exponential_pdf": (d, [d, d]),
+            "or_three_tax_log_like": (i, [C.POINTER(OrCore), C.c_char_p, i, d, d, _dp]),
+            "or_calc_marg_prob": (i, [C.POINTER(OrCore), C.c_char_p, i, d, d, _dp,
+                                      C.POINTER(C.c_long)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -373,11 +381,61 @@ class TreeC:
                         P(self.bl))
 
 
+PRIOR_FLAT_EXP, PRIOR_UNIFORM, PRIOR_INFORMATIVE = 0, 1, 2
+
+
 def default_prefs(**kw):
+    """prefs.ml:65-121 defaults.  calc_pp=True applies -p's changes
+    (prefs.ml:198-203: keep_at_most 20, keep_factor 0.001, max_strikes 20)
+    unless they are given.  prior: "flat" (exponential, mean = tree length /
+    edges, pplacer_run.ml:386-389), "uniform" or "informative" (with
+    prior_lower, core.ml:34-52)."""
     p = dict(start_pend=0.1, max_pend=2.0, initial_tolerance=0.01, strike_box=3.0,
-             keep_factor=0.01, max_strikes=6, max_pitches=40, keep_at_most=7)
+             keep_factor=0.01, max_strikes=6, max_pitches=40, keep_at_most=7,
+             calc_pp=False, pp_rel_err=0.01, prior="flat", prior_lower=0.0)
+    if kw.get("calc_pp"):
+        p.update(keep_at_most=20, keep_factor=0.001, max_strikes=20)
     p.update(kw)
     return p
+
+
+def tree_length(tree) -> float:
+    """Gtree.tree_length (gtree.ml:102-114): every non-root branch length,
+    summed subtree by subtree as the reference folds them."""
+    def aux(i):
+        acc = float(tree.bl[i])
+        for c in tree.children[i]:
+            acc = acc + aux(c)
+        return acc
+    tot = 0.0
+    for c in tree.children[tree.root]:
+        tot = tot + aux(c)
+    return tot
+
+
+def prior_mean_map(tree, lower=0.0):
+    """Core.prior_mean_map (core.ml:34-52) with top_segment_f = (+ lower):
+    per edge id, lower + its branch length + the average distance from its
+    lower node to the leaves below, in the reference's list order."""
+    bl = [float(x) for x in tree.bl]
+    means = np.zeros(tree.n_nodes)
+
+    def aux(i):
+        if not tree.children[i]:
+            means[i] = lower + bl[i]
+            return [bl[i]]
+        dist = []
+        for c in tree.children[i]:
+            dist = dist + aux(c)
+        shifted = [d + (lower + bl[i]) for d in dist]
+        tot = shifted[0]
+        for d in shifted[1:]:
+            tot = tot + d
+        means[i] = tot / float(len(shifted))
+        return [d + bl[i] for d in dist]
+
+    aux(tree.root)
+    return means
 
 
 class Precompute:
@@ -419,12 +477,34 @@ class Core:
         self.pre = pre
         self.prefs = default_prefs(**prefs)
         p = self.prefs
+        tree = pre.tree
+        kind = {"flat": PRIOR_FLAT_EXP, "uniform": PRIOR_UNIFORM,
+                "informative": PRIOR_INFORMATIVE}[p["prior"]]
+        mean = tree_length(tree) / float(tree.n_nodes - 1)
+        self.prior_means = f64(prior_mean_map(tree, p["prior_lower"]))
         self.c = OrCore(C.pointer(pre.model.c), C.pointer(pre.tc.c), pre.N, P(pre.darr),
                         P(pre.darr_e, C.c_int64), P(pre.parr), P(pre.parr_e, C.c_int64),
                         P(pre.snodes), P(pre.snodes_e, C.c_int64),
                         OrPrefs(p["start_pend"], p["max_pend"], p["initial_tolerance"],
                                 p["strike_box"], p["keep_factor"], p["max_strikes"],
-                                p["max_pitches"], p["keep_at_most"]))
+                                p["max_pitches"], p["keep_at_most"], int(bool(p["calc_pp"])),
+                                p["pp_rel_err"], kind, mean, P(self.prior_means)))
+
+    def log_like(self, query, loc, pend, distal):
+        """Three_tax.log_like of the query at (loc, pend, distal)."""
+        ll = C.c_double()
+        st = lib().or_three_tax_log_like(C.byref(self.c), query.encode(), loc, pend, distal,
+                                         C.byref(ll))
+        assert st == 0
+        return ll.value
+
+    def marg_prob(self, query, loc, pend, distal):
+        """Three_tax.calc_marg_prob at (loc, pend, distal): (status, value, calls)."""
+        m = C.c_double()
+        calls = C.c_long(0)
+        st = lib().or_calc_marg_prob(C.byref(self.c), query.encode(), loc, pend, distal,
+                                     C.byref(m), C.byref(calls))
+        return st, m.value, calls.value
 
     def stage1_scores(self, query: str):
         E = self.pre.tree.n_nodes - 1
@@ -455,7 +535,9 @@ class Core:
         stats = OrStats()
         st = lib().or_process_query(C.byref(self.c), query.encode(), out, C.byref(n),
                                     C.byref(stats))
-        pl = [(out[i].loc, out[i].ml_ratio, out[i].log_like, out[i].dist_bl, out[i].pend_bl)
+        pp = self.prefs["calc_pp"]
+        pl = [(out[i].loc, out[i].ml_ratio, out[i].log_like, out[i].dist_bl, out[i].pend_bl,
+               out[i].post_prob if pp else None, out[i].marginal_prob if pp else None)
               for i in range(n.value)]
         return st, pl, stats
 
@@ -487,6 +569,20 @@ def brent(f, raw_start, left, right, tol, max_iters=100):
     res = C.c_double()
     st = lib().or_brent(fn, None, raw_start, left, right, tol, max_iters, C.byref(res))
     return st, res.value, calls
+
+
+def qng(f, a, b, epsabs, epsrel):
+    """or_qng on a Python function: (status, result, abserr, neval, points)."""
+    pts = []
+
+    def cb(x, _ctx):
+        pts.append(x)
+        return f(x)
+
+    fn = OR_FUN(cb)
+    r, e, n = C.c_double(), C.c_double(), C.c_int()
+    st = lib().or_qng(fn, None, a, b, epsabs, epsrel, C.byref(r), C.byref(e), C.byref(n), None)
+    return st, r.value, e.value, n.value, pts
 
 
 def ll_normalized_prob(ll):

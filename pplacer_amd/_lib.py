@@ -16,7 +16,9 @@ PP_FP64, PP_FP32 = 0, 1
 
 Q_STATUS = {0: "ok", 1: "no informative sites", 2: "unknown character", 3: "empty results",
             4: "FindStartFailure", 5: "InvalidStartValues", 6: "assert prox_bl >= 0",
-            7: "query too long for the GPU path"}
+            7: "query too long for the GPU path", 8: "ExceededMaxIter",
+            9: "GSL error"}
+PP_PRIOR_FLAT_EXP, PP_PRIOR_UNIFORM, PP_PRIOR_INFORMATIVE = 0, 1, 2
 
 
 class PPError(RuntimeError):
@@ -25,14 +27,16 @@ class PPError(RuntimeError):
 
 class Placement(C.Structure):
     _fields_ = [("loc", C.c_int), ("ml_ratio", C.c_double), ("log_like", C.c_double),
-                ("dist_bl", C.c_double), ("pend_bl", C.c_double)]
+                ("dist_bl", C.c_double), ("pend_bl", C.c_double), ("post_prob", C.c_double),
+                ("marginal_prob", C.c_double)]
 
 
 class Prefs(C.Structure):
     _fields_ = [("start_pend", C.c_double), ("max_pend", C.c_double),
                 ("initial_tolerance", C.c_double), ("strike_box", C.c_double),
                 ("keep_factor", C.c_double), ("max_strikes", C.c_int),
-                ("max_pitches", C.c_int), ("keep_at_most", C.c_int)]
+                ("max_pitches", C.c_int), ("keep_at_most", C.c_int), ("calc_pp", C.c_int),
+                ("pp_rel_err", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -43,7 +47,7 @@ class Stats(C.Structure):
                 ("n_site_probes_pend", C.c_int64), ("n_site_probes_dist", C.c_int64),
                 ("n_launches", C.c_int64), ("t_table_ms", C.c_double),
                 ("n_site_probes_pend_pitch", C.c_int64), ("n_site_probes_dist_pitch", C.c_int64),
-                ("n_pitch_launches", C.c_int64)]
+                ("n_pitch_launches", C.c_int64), ("t_pp_ms", C.c_double)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -80,6 +84,10 @@ SIGNATURES = {
     "pp_rank_topk": (_i, [_vp, _i, _i, _dp, _i, _ip]),
     "pp_optimize_batch": (_i, [_vp, _i, C.c_char_p, _ip, _ip, _dp, _dp, _d, _d, _i, _dp, _dp,
                                _dp, _ip, _ip]),
+    "pp_marginal_batch": (_i, [_vp, _i, C.c_char_p, _ip, _ip, _dp, _dp, _d, _d, _dp, _ip, _ip]),
+    "pp_set_prior": (_i, [_vp, _i, _i, _dp]),
+    "pp_tree_length": (_i, [_i, _ip, _ip, _dp, _dp]),
+    "pp_prior_mean_map": (_i, [_i, _ip, _ip, _dp, _d, _dp]),
     "pp_place_batch": (_i, [_vp, _i, C.c_char_p, C.POINTER(Prefs), _i, C.POINTER(Placement),
                             _ip, _ip, C.POINTER(Stats)]),
     "pp_queries_upload": (_i, [_vp, _i, C.c_char_p]),

@@ -96,6 +96,11 @@ struct pp_ctx {
   DBuf b_lt;            // stage-1 log-util table [N][n_inf+1][Epad]
   DBuf b_scratch;       // 20-state stage-2 pair scratch
   DBuf b_cat;           // Price-CAT: category of each pre-masked site [N]
+  // -p prior (Core.prior): kind, flat mean or per-edge means; device [G] means
+  int prior_kind = PP_PRIOR_FLAT_EXP;
+  bool prior_set = false;
+  std::vector<double> prior_means;
+  DBuf b_prior;
   std::vector<double> h_ttab;
   // resident queries (sorted by first informative column)
   int Q = 0;
@@ -556,6 +561,23 @@ int launch_opt(pp_ctx *c, OptArgs a, int bd) {
   return launch_opt_b<Real, R, 1024, CAT>(c, a, bd);
 }
 
+template <int R, int MAXBD, bool CAT>
+int launch_marg_b(pp_ctx *c, OptArgs a, int bd) {
+  const size_t lds = stage2_lds<double, R>(c, bd, 0);
+  const void *k = reinterpret_cast<const void *>(&marginal_mw_kernel<R, MAXBD, kSPL, CAT>);
+  const int grid = occupancy_grid<double, R>(k, bd, lds, c->n_cu, a.n_jobs);
+  hipLaunchKernelGGL((marginal_mw_kernel<R, MAXBD, kSPL, CAT>), dim3(grid), dim3(bd), lds, c->stream, a);
+  HIPCHECK(hipGetLastError());
+  return PP_OK;
+}
+
+template <int R, bool CAT = false>
+int launch_marg(pp_ctx *c, OptArgs a, int bd) {
+  if (bd <= 512) return launch_marg_b<R, 512, CAT>(c, a, bd);
+  if (bd <= 768) return launch_marg_b<R, 768, CAT>(c, a, bd);
+  return launch_marg_b<R, 1024, CAT>(c, a, bd);
+}
+
 // 20-state stage 2: one master + 7 worker waves per query, pairs per lane as
 // the longest masked length needs, per-workgroup pair scratch.
 // Sized for the kernel's full-occupancy grid, so that launches with fewer
@@ -602,6 +624,54 @@ int launch_opt_prot(pp_ctx *c, OptArgs a) {
   hipLaunchKernelGGL((optimize_prot_kernel<Real, R>), dim3(grid), dim3(512), lds, c->stream, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
+}
+
+template <int R>
+int launch_marg_prot(pp_ctx *c, OptArgs a) {
+  const size_t lds = ProtLayout::bytes(c->N);
+  const void *k = reinterpret_cast<const void *>(&marginal_prot_kernel<R>);
+  const int grid = occupancy_grid<double, R>(k, 512, lds, c->n_cu, a.n_jobs);
+  a.cap_pairs = prot_cap_pairs(c);
+  int st = prot_scratch<double>(c, k, lds, a.cap_pairs, &a.scratch);
+  if (st) return st;
+  hipLaunchKernelGGL((marginal_prot_kernel<R>), dim3(grid), dim3(512), lds, c->stream, a);
+  HIPCHECK(hipGetLastError());
+  return PP_OK;
+}
+
+// The -p prior on the device: the exponential mean of every edge (the flat
+// prior repeats its mean); checks that one is known for this tree.
+int upload_prior(pp_ctx *c, OptArgs *o) {
+  CHECK(c->prior_set, PP_ESTATE, "no prior for -p: call pp_set_prior (pp_tree_upload installs none)");
+  std::vector<double> m((size_t)c->G, 0.);
+  if (c->prior_kind == PP_PRIOR_INFORMATIVE) {
+    CHECK((int)c->prior_means.size() == c->G, PP_ESTATE, "informative prior: one mean per node is needed");
+    m = c->prior_means;
+  } else if (c->prior_kind == PP_PRIOR_FLAT_EXP) {
+    std::fill(m.begin(), m.end(), c->prior_means[0]);
+  }
+  double *d;
+  ENSURE(c->b_prior, (size_t)c->G, &d);
+  HIPCHECK(hipMemcpyAsync(d, m.data(), m.size() * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  o->prior_kind = c->prior_kind == PP_PRIOR_UNIFORM ? PRIOR_UNIFORM : PRIOR_EXP;
+  o->prior_mean = d;
+  return PP_OK;
+}
+
+int dispatch_marg(pp_ctx *c, OptArgs a, int bd) {
+  a.site_cat = site_cat_ptr(c);
+  if (c->model.S == 20) {
+    if (c->model.R == 4) return launch_marg_prot<4>(c, a);
+    if (c->model.R == 1) return launch_marg_prot<1>(c, a);
+    set_error("n_rates must be 1 or 4 on the GPU path");
+    return PP_EINVAL;
+  }
+  if (c->model.n_cats > 0) return launch_marg<1, true>(c, a, bd);
+  if (c->model.R == 4) return launch_marg<4>(c, a, bd);
+  if (c->model.R == 1) return launch_marg<1>(c, a, bd);
+  set_error("n_rates must be 1 or 4 on the GPU path");
+  return PP_EINVAL;
 }
 
 template <typename Real>
@@ -720,14 +790,22 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   HIPCHECK(hipMemsetAsync(d_stats, 0, 6 * sizeof(unsigned long long), c->stream));
   HIPCHECK(hipMemcpyAsync(d_status, c->qstat.data(), (size_t)Q * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemsetAsync(d_nout, 0, (size_t)std::max(Q, 1) * 4, c->stream));
-  double t_s1 = 0, t_pitch = 0, t_ref = 0;
+  double t_s1 = 0, t_pitch = 0, t_ref = 0, t_pp = 0;
   long long launches = 0, pitch_launches = 0;
-  hipEvent_t ev0, ev1, ev1b, ev2, ev3;
+  OptArgs marg_proto{};
+  std::memset(&marg_proto, 0, sizeof(marg_proto));
+  if (prefs->calc_pp) {
+    st = upload_prior(c, &marg_proto);
+    if (st) return st;
+  }
+  hipEvent_t ev0, ev1, ev1b, ev2, ev3, evp0, evp1;
   HIPCHECK(hipEventCreate(&ev0));
   HIPCHECK(hipEventCreate(&ev1));
   HIPCHECK(hipEventCreate(&ev1b));
   HIPCHECK(hipEventCreate(&ev2));
   HIPCHECK(hipEventCreate(&ev3));
+  HIPCHECK(hipEventCreate(&evp0));
+  HIPCHECK(hipEventCreate(&evp1));
   const bool fp32 = precision == PP_FP32;
   const bool s2_fp32 = fp32 && stage2_fp32_ok(c);
   float t_table = 0;
@@ -838,9 +916,41 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
       if (st) return st;
       launches++;
     }
+    HIPCHECK(hipEventRecord(evp0, c->stream));
+    if (prefs->calc_pp) {  // core.ml:378-392, one job per kept placement
+      for (int b = 0; b < kBuckets; b++) {
+        const int n = boff[b + 1] - boff[b];
+        if (!n) continue;
+        OptArgs o = marg_proto;
+        o.model = c->d_model;
+        o.codes = codes;
+        o.N = N;
+        o.mode = 1;
+        o.n_jobs = n * K;
+        o.K = K;
+        o.qlist = d_qlist + boff[b];
+        o.counter = d_counter + 32 + b;
+        o.darr = c->b_d64.p;
+        o.parr = c->b_p64.p;
+        o.darr_e = reinterpret_cast<const int *>(c->b_de.p);
+        o.parr_e = reinterpret_cast<const int *>(c->b_pe.p);
+        o.bl = reinterpret_cast<const double *>(c->b_bl.p);
+        o.max_pend = prefs->max_pend;
+        o.pp_rel_err = prefs->pp_rel_err;
+        o.kept = d_kept;
+        o.n_keep = d_nkeep;
+        o.status = status;
+        o.n_calls = d_stats;
+        st = dispatch_marg(c, o, block_of_bucket(b));
+        if (st) return st;
+        launches++;
+      }
+    }
+    HIPCHECK(hipEventRecord(evp1, c->stream));
     FinalArgs fa;
     fa.Q = nq;
     fa.K = K;
+    fa.calc_pp = prefs->calc_pp;
     fa.kept = d_kept;
     fa.n_keep = d_nkeep;
     fa.status = status;
@@ -851,15 +961,18 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     launches += 2;
     HIPCHECK(hipEventRecord(ev3, c->stream));
     HIPCHECK(hipEventSynchronize(ev3));
-    float a1 = 0, a2 = 0, a3 = 0;
+    float a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0;
     HIPCHECK(hipEventElapsedTime(&a1, ev0, ev1));
     HIPCHECK(hipEventElapsedTime(&a2, ev1b, ev2));
-    HIPCHECK(hipEventElapsedTime(&a3, ev2, ev3));
+    HIPCHECK(hipEventElapsedTime(&a3, ev2, evp0));
+    HIPCHECK(hipEventElapsedTime(&a4, evp0, evp1));
+    HIPCHECK(hipEventElapsedTime(&a5, evp1, ev3));
     t_s1 += a1;
     t_pitch += a2;
-    t_ref += a3;
+    t_ref += a3 + a5;
+    t_pp += a4;
   }
-  for (hipEvent_t e : {ev0, ev1, ev1b, ev2, ev3}) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {ev0, ev1, ev1b, ev2, ev3, evp0, evp1}) (void)hipEventDestroy(e);
   if (stats) {
     unsigned long long hs[6];
     HIPCHECK(hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost));
@@ -883,6 +996,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     stats->n_site_probes_pend_pitch = (int64_t)hs[4];
     stats->n_site_probes_dist_pitch = (int64_t)hs[5];
     stats->n_pitch_launches = pitch_launches;
+    stats->t_pp_ms = t_pp;
     stats->n_launches = launches;
   }
   return PP_OK;
@@ -950,7 +1064,7 @@ int pp_ctx_destroy(pp_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   DBuf *bufs[] = {&c->b_model, &c->b_bl, &c->b_d64, &c->b_p64, &c->b_d32, &c->b_p32, &c->b_de,
-                  &c->b_pe, &c->b_s64, &c->b_se, &c->b_de32, &c->b_pe32, &c->b_lt, &c->b_scratch, &c->b_cat, &c->b_codes, &c->b_qM, &c->b_tcodes,
+                  &c->b_pe, &c->b_s64, &c->b_se, &c->b_de32, &c->b_pe32, &c->b_lt, &c->b_scratch, &c->b_cat, &c->b_prior, &c->b_codes, &c->b_qM, &c->b_tcodes,
                   &c->b_tile_lo, &c->b_tile_hi, &c->b_tile_q, &c->b_ttab, &c->b_scores, &c->b_res,
                   &c->b_nres, &c->b_status, &c->b_kept, &c->b_nkeep, &c->b_out, &c->b_nout,
                   &c->b_qlist, &c->b_counter, &c->b_stats};
@@ -1032,7 +1146,28 @@ int pp_tree_set(pp_ctx *c, int n_nodes, const int *child_start, const int *child
   c->E = n_nodes - 1;
   c->Epad = (c->E + 255) / 256 * 256;
   c->bl.assign(bl, bl + n_nodes);
+  // the default prior: exponential with mean = tree length / edges
+  // (pplacer_run.ml:386-389)
+  c->prior_kind = PP_PRIOR_FLAT_EXP;
+  c->prior_means.assign(1, tree_length(t) / (double)(n_nodes - 1));
+  c->prior_set = true;
   return gpu_precompute(c, t, leaf_seqs);
+}
+
+int pp_set_prior(pp_ctx *c, int kind, int n, const double *means) {
+  CHECK(c, PP_EINVAL, "null context");
+  CHECK(kind == PP_PRIOR_FLAT_EXP || kind == PP_PRIOR_UNIFORM || kind == PP_PRIOR_INFORMATIVE,
+        PP_EINVAL, "unknown prior kind");
+  if (kind != PP_PRIOR_UNIFORM) {
+    CHECK(means && n >= 1, PP_EINVAL, "the exponential priors need their means");
+    if (kind == PP_PRIOR_FLAT_EXP) n = 1;
+    c->prior_means.assign(means, means + n);
+  } else {
+    c->prior_means.clear();
+  }
+  c->prior_kind = kind;
+  c->prior_set = true;
+  return PP_OK;
 }
 
 int pp_tree_upload(pp_ctx *c, int n_nodes, const double *bl, int n_sites, const double *darr,
@@ -1056,6 +1191,7 @@ int pp_tree_upload(pp_ctx *c, int n_nodes, const double *bl, int n_sites, const 
   c->E = n_nodes - 1;
   c->Epad = (c->E + 255) / 256 * 256;
   c->bl.assign(bl, bl + n_nodes);
+  c->prior_set = false;  // no tree shape: the host sets the prior for -p
   return upload_glvs(c, g);
 }
 
@@ -1260,6 +1396,87 @@ int pp_optimize_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
   HIPCHECK(hipMemcpy(ll, d_ll, n_jobs * 8, hipMemcpyDeviceToHost));
   HIPCHECK(hipMemcpy(pend, d_pe, n_jobs * 8, hipMemcpyDeviceToHost));
   HIPCHECK(hipMemcpy(dist, d_di, n_jobs * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(n_calls, d_calls, n_jobs * 4, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(status, d_st, n_jobs * 4, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+int pp_marginal_batch(pp_ctx *c, int n_jobs, const char *queries, const int *query_of_job,
+                      const int *locs, const double *pend, const double *dist, double rel_err,
+                      double max_pend, double *marg, int *n_calls, int *status) {
+  CHECK(c && queries && query_of_job && locs && pend && dist && marg && n_calls && status,
+        PP_EINVAL, "null argument");
+  int n_queries = 0;
+  for (int j = 0; j < n_jobs; j++) n_queries = std::max(n_queries, query_of_job[j] + 1);
+  int st = pp_queries_upload(c, n_queries, queries);
+  if (st) return st;
+  for (int j = 0; j < n_jobs; j++) CHECK(locs[j] >= 0 && locs[j] < c->E, PP_EINVAL, "loc out of range");
+  OptArgs proto{};
+  std::memset(&proto, 0, sizeof(proto));
+  st = upload_prior(c, &proto);
+  if (st) return st;
+  std::vector<int> pos(n_queries);
+  for (int i = 0; i < n_queries; i++) pos[c->order[i]] = i;
+  std::vector<int> jq(n_jobs);
+  for (int j = 0; j < n_jobs; j++) jq[j] = pos[query_of_job[j]];
+  Scratch s;
+  int *d_jq = s.up(jq.data(), n_jobs), *d_loc = s.up(locs, n_jobs);
+  double *d_p0 = s.up(pend, n_jobs), *d_d0 = s.up(dist, n_jobs);
+  double *d_m = s.up<double>(nullptr, n_jobs);
+  int *d_calls = s.up<int>(nullptr, n_jobs), *d_st = s.up<int>(nullptr, n_jobs);
+  unsigned long long *d_stats = s.up<unsigned long long>(nullptr, 4);
+  int *d_counter = s.up<int>(nullptr, 16);
+  CHECK(d_jq && d_loc && d_p0 && d_d0 && d_m && d_calls && d_st && d_stats && d_counter, PP_ENOMEM,
+        "device allocation failed");
+  std::vector<int> init_st(n_jobs, PP_Q_OK);
+  for (int j = 0; j < n_jobs; j++)
+    if (c->qstat[jq[j]] != PP_Q_OK) init_st[j] = 100 + c->qstat[jq[j]];
+  HIPCHECK(hipMemcpy(d_st, init_st.data(), n_jobs * 4, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemset(d_counter, 0, 64));
+  HIPCHECK(hipMemset(d_stats, 0, 32));
+  std::vector<std::vector<int>> buckets(kBuckets);
+  for (int j = 0; j < n_jobs; j++)
+    if (init_st[j] == PP_Q_OK) buckets[bucket_of(c->qM[jq[j]])].push_back(j);
+  std::vector<int> flat;
+  std::vector<int> boff(kBuckets + 1, 0);
+  for (int b = 0; b < kBuckets; b++) {
+    boff[b] = (int)flat.size();
+    flat.insert(flat.end(), buckets[b].begin(), buckets[b].end());
+  }
+  boff[kBuckets] = (int)flat.size();
+  int *d_list = s.up(flat.data(), flat.size());
+  for (int b = 0; b < kBuckets; b++) {
+    const int n = boff[b + 1] - boff[b];
+    if (!n) continue;
+    OptArgs o = proto;
+    o.model = c->d_model;
+    o.codes = reinterpret_cast<const uint8_t *>(c->b_codes.p);
+    o.N = c->N;
+    o.mode = 0;
+    o.n_jobs = n;
+    o.K = 1;
+    o.qlist = d_list + boff[b];
+    o.counter = d_counter + b;
+    o.darr = c->b_d64.p;
+    o.parr = c->b_p64.p;
+    o.darr_e = reinterpret_cast<const int *>(c->b_de.p);
+    o.parr_e = reinterpret_cast<const int *>(c->b_pe.p);
+    o.bl = reinterpret_cast<const double *>(c->b_bl.p);
+    o.max_pend = max_pend;
+    o.pp_rel_err = rel_err;
+    o.job_q = d_jq;
+    o.job_loc = d_loc;
+    o.job_pend = d_p0;
+    o.job_dist = d_d0;
+    o.out_ll = d_m;
+    o.out_calls = d_calls;
+    o.out_status = d_st;
+    o.n_calls = d_stats;
+    st = dispatch_marg(c, o, block_of_bucket(b));
+    if (st) return st;
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipMemcpy(marg, d_m, n_jobs * 8, hipMemcpyDeviceToHost));
   HIPCHECK(hipMemcpy(n_calls, d_calls, n_jobs * 4, hipMemcpyDeviceToHost));
   HIPCHECK(hipMemcpy(status, d_st, n_jobs * 4, hipMemcpyDeviceToHost));
   return PP_OK;

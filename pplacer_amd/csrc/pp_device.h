@@ -22,12 +22,15 @@ namespace pp {
 
 enum {
   ST_OK = 0,
-  ST_GSL = 1,          // Gsl_exn (EBADFUNC / EINVAL)
+  ST_GSL = 1,          // Gsl_exn: EINVAL from the minimiser's set, or EBADTOL
+  ST_GSL_BADFUNC = 2,  // Gsl_exn: EBADFUNC, a non-finite objective value
   ST_MAXITER = 3,      // Minimization.ExceededMaxIter
   ST_FIND_START = 4,   // Minimization.FindStartFailure
   ST_INVALID_START = 5,// Minimization.InvalidStartValues
   ST_ASSERT = 6        // assert(prox_bl >= 0.)
 };
+
+__device__ __forceinline__ bool is_gsl(int st) { return st == ST_GSL || st == ST_GSL_BADFUNC; }
 
 // Stable descending order of stage-1 scores as a 64-bit ascending key
 // (fig.ml:143-147 with OCaml float compare: nan below everything, -0 == 0).

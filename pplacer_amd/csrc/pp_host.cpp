@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "pp_internal.h"
+#include "pp_tree.h"
 
 namespace pp {
 
@@ -139,6 +140,49 @@ static void dediag(double *dst, const double *u, const double *lambda, const dou
       for (int k = 0; k < n; k++) acc += lambda[k] * u[i * n + k] * uit[j * n + k];
       dst[i * n + j] = acc;
     }
+}
+
+// ---- tree summaries for the -p priors --------------------------------------
+
+static double subtree_length(const TreeDesc &t, int i) {
+  double acc = t.bl[i];  // fold_left (+.) (bl id) over the children
+  for (int c = t.child_start[i]; c < t.child_start[i + 1]; c++) acc = acc + subtree_length(t, t.children[c]);
+  return acc;
+}
+
+double tree_length(const TreeDesc &t) {
+  const int root = t.n_nodes - 1;
+  double tot = 0.;
+  for (int c = t.child_start[root]; c < t.child_start[root + 1]; c++)
+    tot = tot + subtree_length(t, t.children[c]);
+  return tot;
+}
+
+// returns the distances from the top of edge i to the leaves below, in order
+static std::vector<double> mean_map_aux(const TreeDesc &t, int i, double lower, double *means) {
+  const double bl = t.bl[i];
+  if (t.child_start[i] == t.child_start[i + 1]) {
+    means[i] = lower + bl;
+    return {bl};
+  }
+  std::vector<double> dist;
+  for (int c = t.child_start[i]; c < t.child_start[i + 1]; c++) {
+    std::vector<double> d = mean_map_aux(t, t.children[c], lower, means);
+    dist.insert(dist.end(), d.begin(), d.end());
+  }
+  const double top = lower + bl;
+  double tot = top + dist[0];  // List.reduce (+.) over the shifted list
+  for (size_t k = 1; k < dist.size(); k++) tot = tot + (top + dist[k]);
+  means[i] = tot / (double)dist.size();
+  for (double &d : dist) d = bl + d;
+  return dist;
+}
+
+void prior_mean_map(const TreeDesc &t, double lower, double *means) {
+  for (int i = 0; i < t.n_nodes; i++) means[i] = 0.;
+  const int root = t.n_nodes - 1;
+  for (int c = t.child_start[root]; c < t.child_start[root + 1]; c++)
+    (void)mean_map_aux(t, t.children[c], lower, means);
 }
 
 // diagd.ml:79-100; with Price-CAT one matrix per category (gcat_model.ml:70-82)
@@ -358,6 +402,28 @@ int pp_premask(int n_states, const char *ref_seqs, int n_ref, const char *query_
   return PP_OK;
 }
 
+int pp_tree_length(int n_nodes, const int *child_start, const int *children, const double *bl,
+                   double *length) {
+  if (!child_start || !children || !bl || !length || n_nodes < 1) {
+    pp::set_error("bad arguments");
+    return PP_EINVAL;
+  }
+  pp::TreeDesc t{n_nodes, 0, child_start, children, bl};
+  *length = pp::tree_length(t);
+  return PP_OK;
+}
+
+int pp_prior_mean_map(int n_nodes, const int *child_start, const int *children,
+                      const double *bl, double lower, double *means) {
+  if (!child_start || !children || !bl || !means || n_nodes < 1) {
+    pp::set_error("bad arguments");
+    return PP_EINVAL;
+  }
+  pp::TreeDesc t{n_nodes, 0, child_start, children, bl};
+  pp::prior_mean_map(t, lower, means);
+  return PP_OK;
+}
+
 void pp_prefs_default(pp_prefs *p) {
   p->start_pend = 0.1;
   p->max_pend = 2.0;
@@ -367,6 +433,8 @@ void pp_prefs_default(pp_prefs *p) {
   p->max_strikes = 6;
   p->max_pitches = 40;
   p->keep_at_most = 7;
+  p->calc_pp = 0;
+  p->pp_rel_err = 0.01;
 }
 
 }  // extern "C"

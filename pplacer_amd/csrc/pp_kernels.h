@@ -13,6 +13,7 @@ struct Res {  // one optimised (query, edge): Three_tax.get_results + location
   double ll, pend, dist;
   int loc;
   int flags;
+  double marg;  // -p: the marginal likelihood (Three_tax.calc_marg_prob)
 };
 
 struct CodeLogArgs {
@@ -94,10 +95,15 @@ struct OptArgs {
   void *scratch;                // 20-state engine, as in PitchArgs
   int cap_pairs;
   const int *site_cat;          // as in PitchArgs
+  // marginal likelihoods (-p): mode 1 writes kept[].marg, mode 0 out_ll
+  double pp_rel_err;
+  int prior_kind;               // PRIOR_EXP or PRIOR_UNIFORM
+  const double *prior_mean;     // [G] exponential prior mean of each edge
 };
 
 struct FinalArgs {
   int Q, K;
+  int calc_pp;  // post_prob / marginal_prob from kept[].marg
   const Res *kept;
   const int *n_keep;
   const int *status;

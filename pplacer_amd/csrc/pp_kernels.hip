@@ -202,7 +202,7 @@ __device__ __forceinline__ void pitch_loop(const PitchArgs &a, Master &m, uint8_
       const int st = m.safe_optimize(a.initial_tol, a.max_pend, a.start_pend, cut, &ll, &pend, &dist);
       calls += m.n_calls;
       dprobes += m.n_dist_probes;
-      if (st == ST_GSL) continue;  // "GSL problem with location": skipped (core.ml:286-297)
+      if (is_gsl(st)) continue;  // "GSL problem with location": skipped (core.ml:286-297)
       if (st != ST_OK && st != ST_MAXITER) {
         status = st == ST_FIND_START ? PP_Q_FIND_START
                  : st == ST_INVALID_START ? PP_Q_INVALID_START : PP_Q_ASSERT;
@@ -324,7 +324,7 @@ __device__ __forceinline__ void optimize_loop(const OptArgs &a, Master &m) {
           slot->ll = ll;
           slot->pend = pend;
           slot->dist = dist;
-        } else if (st != ST_GSL) {
+        } else if (!is_gsl(st)) {
           a.status[q] = st == ST_FIND_START ? PP_Q_FIND_START
                         : st == ST_INVALID_START ? PP_Q_INVALID_START : PP_Q_ASSERT;
         }  // a GSL problem in the final optimisation keeps the initial result
@@ -332,6 +332,71 @@ __device__ __forceinline__ void optimize_loop(const OptArgs &a, Master &m) {
         a.out_ll[jj] = ll;
         a.out_pend[jj] = pend;
         a.out_dist[jj] = dist;
+        a.out_calls[jj] = (int)m.n_calls;
+        a.out_status[jj] = st;
+      }
+    }
+  }
+  m.command(CMD_EXIT);
+}
+
+// Marginal likelihoods for -p (core.ml:378-392): per kept placement (mode 1)
+// or per job (mode 0), Three_tax.calc_marg_prob from the placement's branch
+// lengths with the prior of its edge.  FP64 only, as the final likelihoods.
+template <class Master>
+__device__ __forceinline__ void marginal_loop(const OptArgs &a, Master &m) {
+  const int lane = m.lane;
+#pragma unroll 1
+  for (;;) {
+    int job = 0;
+    if (lane == 0) job = atomicAdd(a.counter, 1);
+    job = __builtin_amdgcn_readfirstlane(job);
+    if (job >= a.n_jobs) break;
+    int q, loc;
+    double pend0, dist0;
+    Res *slot = nullptr;
+    int jj = 0;
+    if (a.mode == 1) {
+      const int qi = job / a.K, k = job % a.K;
+      q = a.qlist[qi];
+      if (a.status[q] != PP_Q_OK || k >= a.n_keep[q]) continue;
+      slot = a.kept + (size_t)q * a.K + k;
+      loc = slot->loc;
+      pend0 = slot->pend;
+      dist0 = slot->dist;
+    } else {
+      jj = a.qlist ? a.qlist[job] : job;
+      q = a.job_q[jj];
+      loc = a.job_loc[jj];
+      pend0 = a.job_pend[jj];
+      dist0 = a.job_dist[jj];
+    }
+    const int M = m.load_query(a.codes + (size_t)q * a.N, a.N, q);
+    if (M == 0 || M > m.cap_sites) {
+      if (lane == 0 && a.mode == 0) a.out_status[jj] = M == 0 ? PP_Q_NO_INFORMATIVE : PP_Q_RANK_EXHAUSTED;
+      continue;
+    }
+    m.gather(loc);
+    m.set_state(a.bl[loc], pend0, dist0);  // tt_edges_from_placement (core.ml:241-250)
+    m.pr_kind = a.prior_kind;
+    m.pr_mu = a.prior_mean[loc];
+    m.n_calls = 0;
+    m.n_dist_probes = 0;
+    double marg = 0.;
+    const int st = m.calc_marg_prob(a.pp_rel_err, a.max_pend, &marg);
+    if (lane == 0) {
+      atomicAdd(a.n_calls, (unsigned long long)m.n_calls);
+      atomicAdd(a.n_calls + 2, (unsigned long long)((m.n_calls - m.n_dist_probes) * M));
+      atomicAdd(a.n_calls + 3, (unsigned long long)(m.n_dist_probes * M));
+      if (a.mode == 1) {
+        if (st == ST_OK) slot->marg = marg;
+        else  // uncaught in the reference: the query fails
+          a.status[q] = st == ST_MAXITER ? PP_Q_MAXITER
+                        : st == ST_FIND_START ? PP_Q_FIND_START
+                        : st == ST_INVALID_START ? PP_Q_INVALID_START
+                        : st == ST_ASSERT ? PP_Q_ASSERT : PP_Q_GSL;
+      } else {
+        a.out_ll[jj] = marg;
         a.out_calls[jj] = (int)m.n_calls;
         a.out_status[jj] = st;
       }
@@ -373,6 +438,40 @@ __global__ __launch_bounds__(512) void optimize_prot_kernel(OptArgs a) {
   ProtMaster<Real, R> m;
   m.init(smem, a.model, a.cap_pairs);
   optimize_loop(a, m);
+}
+
+template <int R, int MAXBD, int SPL, bool CAT>
+__global__ __launch_bounds__(MAXBD) void marginal_mw_kernel(OptArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  const MWLayout<double, R> lay = mw_layout<double, R>(a.model);
+  mw_copy_model(reinterpret_cast<MWShared *>(smem), a.model);
+  __syncthreads();
+  if (wave_id() != 0) {
+    char *wl = smem + lay.align(lay.bytes(a.N));
+    mw_worker<double, R, SPL, CAT>(smem, lay, wl, a.N, a.codes, reinterpret_cast<const double *>(a.darr),
+                                   reinterpret_cast<const double *>(a.parr), a.darr_e, a.parr_e,
+                                   a.site_cat);
+    return;
+  }
+  MWMaster<double, R, SPL, CAT> m;
+  m.init(smem, a.model);
+  marginal_loop(a, m);
+}
+
+template <int R>
+__global__ __launch_bounds__(512) void marginal_prot_kernel(OptArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  prot_copy_model(reinterpret_cast<ProtShared *>(smem), a.model);
+  __syncthreads();
+  if (wave_id() != 0) {
+    double *scr = reinterpret_cast<double *>(a.scratch) + ProtLayout::scratch_reals(a.cap_pairs) * blockIdx.x;
+    prot_worker<double, R>(smem, scr, a.cap_pairs, a.N, a.codes, reinterpret_cast<const double *>(a.darr),
+                           reinterpret_cast<const double *>(a.parr), a.darr_e, a.parr_e, a.site_cat);
+    return;
+  }
+  ProtMaster<double, R> m;
+  m.init(smem, a.model, a.cap_pairs);
+  marginal_loop(a, m);
 }
 
 // ---------------------------------------------------------------------------
@@ -439,6 +538,15 @@ __global__ void finalize_kernel(FinalArgs a) {
     while (j >= 0 && desc_key(ratio[order[j]]) > desc_key(ratio[v])) { order[j + 1] = order[j]; j--; }
     order[j + 1] = v;
   }
+  // -p: posterior probabilities = ll_normalized_prob of the marginal
+  // likelihoods in kept order (core.ml:393-401)
+  double post[64];
+  for (int i = 0; i < n; i++) {
+    double tot = 0.;
+    if (a.calc_pp)
+      for (int j = 0; j < n; j++) tot = tot + exp(k[j].marg - k[i].marg);
+    post[i] = a.calc_pp ? 1. / tot : __builtin_nan("");
+  }
   pp_placement *out = a.out + (size_t)q * a.K;
   for (int i = 0; i < n; i++) {
     const Res &r = k[order[i]];
@@ -447,6 +555,8 @@ __global__ void finalize_kernel(FinalArgs a) {
     out[i].log_like = r.ll;
     out[i].dist_bl = r.dist;
     out[i].pend_bl = r.pend;
+    out[i].post_prob = post[order[i]];
+    out[i].marginal_prob = a.calc_pp ? r.marg : __builtin_nan("");
   }
   a.n_out[q] = n;
 }

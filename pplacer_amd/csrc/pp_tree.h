@@ -20,5 +20,12 @@ struct TreeGlvs {
   std::vector<int64_t> darr_e, parr_e, snodes_e;
 };
 
+// Gtree.tree_length (gtree.ml:102-114): the non-root branch lengths, folded
+// subtree by subtree in the reference's order.
+double tree_length(const TreeDesc &t);
+// Core.prior_mean_map with top_segment_f = (+ lower) (core.ml:34-52):
+// means[i] = lower + bl[i] + the average distance from node i to the leaves
+// below it, summed in leaf order; the root entry is left at 0.
+void prior_mean_map(const TreeDesc &t, double lower, double *means);
 
 }  // namespace pp

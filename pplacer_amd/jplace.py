@@ -82,14 +82,22 @@ def tree_string(tree: refio.Tree) -> str:
     return rec(tree.root) + ";"
 
 
+PP_FIELDS = ["marginal_like", "post_prob"]
+
+
 def to_json(tree: refio.Tree, pqueries, invocation: str = "pplacer_amd") -> dict:
-    """pqueries: [(namlom, [PlacementRecord sorted by like_weight_ratio])]."""
+    """pqueries: [(namlom, [PlacementRecord sorted by like_weight_ratio])].
+    Fields are the sorted union of the placements' fields
+    (placerun_io.ml:77-83): post_prob and marginal_like join them when any
+    placement carries posterior probabilities (placement.ml:176-180)."""
+    has_pp = any(r.post_prob is not None for _, pl in pqueries for r in pl)
+    fields = sorted(FIELDS + (PP_FIELDS if has_pp else []))
     placements = []
     for namlom, pl in pqueries:
-        p = [[getattr(r, f) for f in FIELDS] for r in pl]
+        p = [[getattr(r, f) for f in fields] for r in pl]
         placements.append({"p": p, "nm": [[n, float(m)] for n, m in namlom]})
     return {"tree": tree_string(tree), "placements": placements,
-            "metadata": {"invocation": invocation}, "version": 3, "fields": FIELDS}
+            "metadata": {"invocation": invocation}, "version": 3, "fields": fields}
 
 
 def write(path: str, tree: refio.Tree, pqueries, invocation: str = "pplacer_amd") -> None:

@@ -14,7 +14,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import refio
-from ._lib import (PP_FP32, PP_FP64, Placement, Prefs, Stats, check, lib)
+from ._lib import (PP_FP32, PP_FP64, PP_PRIOR_FLAT_EXP, PP_PRIOR_INFORMATIVE, PP_PRIOR_UNIFORM,
+                   Placement, Prefs, Stats, check, lib)
 
 _dp = C.POINTER(C.c_double)
 
@@ -30,12 +31,35 @@ def _p(a, t=C.c_double):
 def default_prefs(**kw) -> Prefs:
     """prefs.ml:65-121 defaults (start_pend 0.1, max_pend 2, initial_tolerance
     0.01, strike_box 3, keep_factor 0.01, max_strikes 6, max_pitches 40,
-    keep_at_most 7)."""
+    keep_at_most 7, pp_rel_err 0.01).  calc_pp=True applies the rest of -p
+    (prefs.ml:198-203: keep_at_most 20, keep_factor 0.001, max_strikes 20)
+    unless those are given too."""
     p = Prefs()
     lib().pp_prefs_default(C.byref(p))
+    if kw.get("calc_pp"):
+        p.keep_at_most, p.keep_factor, p.max_strikes = 20, 0.001, 20
     for k, v in kw.items():
-        setattr(p, k, v)
+        setattr(p, k, int(v) if k == "calc_pp" else v)
     return p
+
+
+def tree_length(tree: refio.Tree) -> float:
+    """Gtree.tree_length (gtree.ml:102-114)."""
+    starts, children = tree.csr()
+    out = C.c_double()
+    check(lib().pp_tree_length(tree.n_nodes, _p(starts, C.c_int), _p(children, C.c_int),
+                               _p(_f64(tree.bl)), C.byref(out)), "tree_length")
+    return out.value
+
+
+def prior_mean_map(tree: refio.Tree, lower: float = 0.0) -> np.ndarray:
+    """Core.prior_mean_map with top_segment_f = (+ lower) (core.ml:34-52):
+    the informative prior's mean for every edge id."""
+    starts, children = tree.csr()
+    out = np.empty(tree.n_nodes)
+    check(lib().pp_prior_mean_map(tree.n_nodes, _p(starts, C.c_int), _p(children, C.c_int),
+                                  _p(_f64(tree.bl)), lower, _p(out)), "prior_mean_map")
+    return out
 
 
 def discrete_gamma(n_cat: int, alpha: float) -> np.ndarray:
@@ -76,13 +100,16 @@ def cut_columns(seqs, mask):
 
 @dataclass
 class PlacementRecord:
-    """Placement.make_ml fields (placement.ml:50-60)."""
+    """Placement.make_ml fields (placement.ml:50-60); with -p also
+    post_prob and marginal_like (placement.ml:62-65, 176-180)."""
 
     edge_num: int
     like_weight_ratio: float
     likelihood: float
     distal_length: float
     pendant_length: float
+    post_prob: float | None = None
+    marginal_like: float | None = None
 
 
 class Placer:
@@ -213,13 +240,47 @@ class Placer:
               "results_download")
         res = []
         for q in range(Q):
-            res.append([PlacementRecord(out[q * K + i].loc, out[q * K + i].ml_ratio,
-                                        out[q * K + i].log_like, out[q * K + i].dist_bl,
-                                        out[q * K + i].pend_bl) for i in range(n_out[q])])
+            row = []
+            for i in range(n_out[q]):
+                o = out[q * K + i]
+                pp = (o.post_prob, o.marginal_prob) if prefs.calc_pp else (None, None)
+                row.append(PlacementRecord(o.loc, o.ml_ratio, o.log_like, o.dist_bl, o.pend_bl,
+                                           *pp))
+            res.append(row)
         if strict and np.any(status[:Q] != 0):
             bad = int(np.nonzero(status[:Q])[0][0])
             raise RuntimeError("query %d failed with status %d" % (bad, status[bad]))
         return res, status[:Q].copy(), stats
+
+    def set_prior(self, kind: str = "flat", lower: float = 0.0):
+        """The -p prior as pplacer_run.ml:380-389 chooses it: "flat"
+        (exponential, mean = tree length / edges; the default pp_tree_set
+        installs), "uniform" (--unif-prior) or "informative" (--inform-prior,
+        with --prior-lower)."""
+        if kind == "uniform":
+            check(lib().pp_set_prior(self._ctx, PP_PRIOR_UNIFORM, 0, None), "set_prior")
+        elif kind == "flat":
+            m = np.array([tree_length(self.tree) / float(self.tree.n_nodes - 1)])
+            check(lib().pp_set_prior(self._ctx, PP_PRIOR_FLAT_EXP, 1, _p(m)), "set_prior")
+        elif kind == "informative":
+            m = prior_mean_map(self.tree, lower)
+            check(lib().pp_set_prior(self._ctx, PP_PRIOR_INFORMATIVE, m.shape[0], _p(m)),
+                  "set_prior")
+        else:
+            raise ValueError("unknown prior %r" % kind)
+
+    def marginal(self, queries, query_of_job, locs, pend, dist, rel_err=0.01, max_pend=2.0):
+        """Three_tax.calc_marg_prob per job: (marginal log-likelihoods, calls, status)."""
+        n = len(locs)
+        qj = np.ascontiguousarray(query_of_job, np.int32)
+        lc = np.ascontiguousarray(locs, np.int32)
+        pe, di = _f64(pend), _f64(dist)
+        m = np.empty(n)
+        calls, st = np.empty(n, np.int32), np.empty(n, np.int32)
+        check(lib().pp_marginal_batch(self._ctx, n, "".join(queries).encode(), _p(qj, C.c_int),
+                                      _p(lc, C.c_int), _p(pe), _p(di), rel_err, max_pend, _p(m),
+                                      _p(calls, C.c_int), _p(st, C.c_int)), "marginal_batch")
+        return m, calls, st
 
     def upload_queries(self, queries):
         check(lib().pp_queries_upload(self._ctx, len(queries), "".join(queries).encode()),

@@ -45,7 +45,7 @@ def cat_sites(site_categories, mask):
 
 
 def run_placements(tree: refio.Tree, refs, queries, pm: refio.PhyloModel, exch_table=None,
-                   prefs=None, precision=placer.PP_FP64, device=0):
+                   prefs=None, precision=placer.PP_FP64, device=0, prior="flat", prior_lower=0.0):
     """Returns (pqueries, status) where pqueries is [(namlom, [PlacementRecord])]."""
     S = 4 if pm.subs_model == "GTR" else 20
     ref_seqs = [s for _, s in refs]
@@ -71,6 +71,8 @@ def run_placements(tree: refio.Tree, refs, queries, pm: refio.PhyloModel, exch_t
         else:
             pl.build_model(exch, statd, placer.discrete_gamma(pm.n_cats, pm.alpha))
         pl.set_tree(tree, leaf, n)
+        if prefs is not None and prefs.calc_pp:
+            pl.set_prior(prior, prior_lower)  # pplacer_run.ml:380-389
         res, status, stats = pl.place([s.upper() for _, s in uniq], prefs=prefs,
                                       precision=precision)
     finally:
@@ -97,7 +99,22 @@ def main(argv=None):
     ap.add_argument("--keep-factor", type=float, default=0.01)
     ap.add_argument("--start-pend", type=float, default=0.1)
     ap.add_argument("--max-pend", type=float, default=2.0)
+    ap.add_argument("-p", dest="calc_pp", action="store_true",
+                    help="posterior probabilities (sets keep-at-most 20, keep-factor 0.001, "
+                         "max-strikes 20 unless given)")
+    ap.add_argument("--pp-rel-err", type=float, default=0.01)
+    ap.add_argument("--unif-prior", action="store_true")
+    ap.add_argument("--inform-prior", action="store_true")
+    ap.add_argument("--prior-lower", type=float, default=0.0)
     args = ap.parse_args(argv)
+    given = set(a.split("=")[0] for a in (argv if argv is not None else sys.argv[1:]))
+    if args.calc_pp:  # prefs.ml:198-203
+        if "--keep-at-most" not in given:
+            args.keep_at_most = 20
+        if "--keep-factor" not in given:
+            args.keep_factor = 0.001
+        if "--max-strikes" not in given:
+            args.max_strikes = 20
     tree = refio.read_newick(args.tree)
     refs, queries = partition(refio.read_fasta(args.aln), tree)
     pm = refio.read_phylo_model(args.model)
@@ -105,9 +122,12 @@ def main(argv=None):
     prefs = placer.default_prefs(max_strikes=args.max_strikes, strike_box=args.strike_box,
                                  max_pitches=args.max_pitches, keep_at_most=args.keep_at_most,
                                  keep_factor=args.keep_factor, start_pend=args.start_pend,
-                                 max_pend=args.max_pend)
+                                 max_pend=args.max_pend, calc_pp=args.calc_pp,
+                                 pp_rel_err=args.pp_rel_err)
+    prior = "uniform" if args.unif_prior else "informative" if args.inform_prior else "flat"
     pqueries, status = run_placements(tree, refs, queries, pm, exch_table, prefs,
-                                      placer.PP_FP32 if args.fp32 else placer.PP_FP64)
+                                      placer.PP_FP32 if args.fp32 else placer.PP_FP64,
+                                      prior=prior, prior_lower=args.prior_lower)
     jplace.write(args.out, tree, pqueries, invocation=" ".join(sys.argv))
     bad = int(np.sum(np.asarray(status) != 0))
     if bad:
