@@ -45,8 +45,7 @@ namespace {
 
 constexpr int kTQ = 16;           // queries per stage-1 tile
 constexpr int kChunk = 32768;     // queries per device batch
-constexpr int kSPL = 2;           // masked sites per worker lane in stage 2
-constexpr int kBuckets = 15;      // masked-length classes of 64*kSPL sites
+constexpr int kBuckets = 30;      // stage-2 masked-length classes of 64 sites
 
 struct DBuf {
   void *p = nullptr;
@@ -397,8 +396,15 @@ int gpu_precompute(pp_ctx *c, const TreeDesc &t, const char *leaf_seqs) {
   return PP_EINVAL;
 }
 
-int bucket_of(int M) { return M <= 0 ? 0 : (M - 1) / (64 * kSPL); }  // workers = bucket+1
-int block_of_bucket(int b) { return 64 * (b + 2); }                      // + one master wave
+int bucket_of(int M) { return M <= 0 ? 0 : (M - 1) / 64; }
+
+// Masked sites per worker lane in stage 2 for bucket b.  The FP64 kernels
+// hold one site's GLV rows per lane (two sites' rows spill at the 4 waves per
+// SIMD the kernels are built for); queries beyond 15 worker waves of one site
+// take two.  The FP32 kernels always hold two.
+int spl_for(bool fp64, int b) { return fp64 && b < 15 ? 1 : 2; }
+// one master wave + ceil(64 (b + 1) / (64 spl)) worker waves
+int block_for(int b, int spl) { return 64 * (1 + (b + spl) / spl); }
 
 // The stage-1 log-util table for one start_pend (see code_log_kernel).
 // Built from the FP64 supernodes for both forms; the FP32 form stores it in
@@ -495,15 +501,15 @@ int occupancy_grid(const void *kern, int bd, size_t lds, int n_cu, int n_items) 
   return std::max(1, std::min(n_items, nb * n_cu));
 }
 
-// Stage-2 kernels: one master wave + ceil(M/(64*kSPL)) worker waves.  Three
-// launch-bound classes give the register allocator the budget of the
-// occupancy the block size allows (512 lanes: 2 waves/SIMD, 768: 3, 1024: 4).
+// Stage-2 kernels: one master wave + ceil(M / (64 spl)) worker waves.  Three
+// launch-bound classes (512, 768, 1024 lanes) bound the block size; the
+// kernels are built for 4 waves per SIMD (MWOcc).
 // LDS: header and tables, masked-site list, the pitch kernel's per-edge
 // flags, cached products of the worker lanes' sites.
 template <typename Real, int R>
-size_t stage2_lds(const pp_ctx *c, int bd, int taken_bytes) {
+size_t stage2_lds(const pp_ctx *c, int bd, int taken_bytes, int spl) {
   const MWLayout<Real, R> lay(c->model.n_cats > 0 ? c->model.n_cats : 1, c->model.n_sl());
-  return lay.align(lay.bytes(c->N) + (size_t)taken_bytes) + MWProducts<Real, R * 4>::bytes((bd - 64) * kSPL);
+  return lay.align(lay.bytes(c->N) + (size_t)taken_bytes) + MWProducts<Real, R * 4>::bytes((bd - 64) * spl);
 }
 
 // Whether the FP32 form runs stage 2 in FP32.  The 20-state engine evaluates
@@ -519,63 +525,92 @@ int max_masked_sites(const pp_ctx *c) {
   if (c->model.S == 20) return c->N;  // pair state lives in global scratch
   int m = 0;
   for (int b = 0; b < kBuckets; b++) {
-    const size_t lds = c->model.R == 4 ? stage2_lds<double, 4>(c, block_of_bucket(b), c->E)
-                                       : stage2_lds<double, 1>(c, block_of_bucket(b), c->E);
+    const int spl = spl_for(true, b), bd = block_for(b, spl);
+    if (bd > 1024) break;
+    const size_t lds = c->model.R == 4 ? stage2_lds<double, 4>(c, bd, c->E, spl)
+                                       : stage2_lds<double, 1>(c, bd, c->E, spl);
     if (lds > c->max_lds) break;
-    m = 64 * kSPL * (b + 1);
+    m = 64 * (b + 1);
   }
   return m;
 }
 
-template <typename Real, int R, int MAXBD, bool CAT>
+template <typename Real, int R, int MAXBD, int SPL, bool CAT>
 int launch_pitch_b(pp_ctx *c, PitchArgs a, int bd) {
-  const size_t lds = stage2_lds<Real, R>(c, bd, c->E);
-  const void *k = reinterpret_cast<const void *>(&pitch_mw_kernel<Real, R, MAXBD, kSPL, CAT>);
+  const size_t lds = stage2_lds<Real, R>(c, bd, c->E, SPL);
+  const void *k = reinterpret_cast<const void *>(&pitch_mw_kernel<Real, R, MAXBD, SPL, CAT>);
   const int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.nq);
-  hipLaunchKernelGGL((pitch_mw_kernel<Real, R, MAXBD, kSPL, CAT>), dim3(grid), dim3(bd), lds, c->stream, a);
+  hipLaunchKernelGGL((pitch_mw_kernel<Real, R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->stream, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
 
-template <typename Real, int R, int MAXBD, bool CAT>
+template <typename Real, int R, int MAXBD, int SPL, bool CAT>
 int launch_opt_b(pp_ctx *c, OptArgs a, int bd) {
-  const size_t lds = stage2_lds<Real, R>(c, bd, 0);
-  const void *k = reinterpret_cast<const void *>(&optimize_mw_kernel<Real, R, MAXBD, kSPL, CAT>);
+  const size_t lds = stage2_lds<Real, R>(c, bd, 0, SPL);
+  const void *k = reinterpret_cast<const void *>(&optimize_mw_kernel<Real, R, MAXBD, SPL, CAT>);
   const int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.n_jobs);
-  hipLaunchKernelGGL((optimize_mw_kernel<Real, R, MAXBD, kSPL, CAT>), dim3(grid), dim3(bd), lds, c->stream, a);
+  hipLaunchKernelGGL((optimize_mw_kernel<Real, R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->stream, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
 
+// Bucket b's launch: FP64 with one site per lane up to 15 worker waves, two
+// beyond (blocks of more than 512 lanes); FP32 with two.
 template <typename Real, int R, bool CAT = false>
-int launch_pitch(pp_ctx *c, PitchArgs a, int bd) {
-  if (bd <= 512) return launch_pitch_b<Real, R, 512, CAT>(c, a, bd);
-  if (bd <= 768) return launch_pitch_b<Real, R, 768, CAT>(c, a, bd);
-  return launch_pitch_b<Real, R, 1024, CAT>(c, a, bd);
+int launch_pitch(pp_ctx *c, PitchArgs a, int b) {
+  const int spl = spl_for(sizeof(Real) == 8, b), bd = block_for(b, spl);
+  if (spl == 1) {
+    if constexpr (sizeof(Real) == 8) {
+      if (bd <= 512) return launch_pitch_b<Real, R, 512, 1, CAT>(c, a, bd);
+      if (bd <= 768) return launch_pitch_b<Real, R, 768, 1, CAT>(c, a, bd);
+      return launch_pitch_b<Real, R, 1024, 1, CAT>(c, a, bd);
+    }
+  }
+  if constexpr (sizeof(Real) == 4) {
+    if (bd <= 512) return launch_pitch_b<Real, R, 512, 2, CAT>(c, a, bd);
+  }
+  if (bd <= 768) return launch_pitch_b<Real, R, 768, 2, CAT>(c, a, bd);
+  return launch_pitch_b<Real, R, 1024, 2, CAT>(c, a, bd);
 }
 
 template <typename Real, int R, bool CAT = false>
-int launch_opt(pp_ctx *c, OptArgs a, int bd) {
-  if (bd <= 512) return launch_opt_b<Real, R, 512, CAT>(c, a, bd);
-  if (bd <= 768) return launch_opt_b<Real, R, 768, CAT>(c, a, bd);
-  return launch_opt_b<Real, R, 1024, CAT>(c, a, bd);
+int launch_opt(pp_ctx *c, OptArgs a, int b) {
+  const int spl = spl_for(sizeof(Real) == 8, b), bd = block_for(b, spl);
+  if (spl == 1) {
+    if constexpr (sizeof(Real) == 8) {
+      if (bd <= 512) return launch_opt_b<Real, R, 512, 1, CAT>(c, a, bd);
+      if (bd <= 768) return launch_opt_b<Real, R, 768, 1, CAT>(c, a, bd);
+      return launch_opt_b<Real, R, 1024, 1, CAT>(c, a, bd);
+    }
+  }
+  if constexpr (sizeof(Real) == 4) {
+    if (bd <= 512) return launch_opt_b<Real, R, 512, 2, CAT>(c, a, bd);
+  }
+  if (bd <= 768) return launch_opt_b<Real, R, 768, 2, CAT>(c, a, bd);
+  return launch_opt_b<Real, R, 1024, 2, CAT>(c, a, bd);
 }
 
-template <int R, int MAXBD, bool CAT>
+template <int R, int MAXBD, int SPL, bool CAT>
 int launch_marg_b(pp_ctx *c, OptArgs a, int bd) {
-  const size_t lds = stage2_lds<double, R>(c, bd, 0);
-  const void *k = reinterpret_cast<const void *>(&marginal_mw_kernel<R, MAXBD, kSPL, CAT>);
+  const size_t lds = stage2_lds<double, R>(c, bd, 0, SPL);
+  const void *k = reinterpret_cast<const void *>(&marginal_mw_kernel<R, MAXBD, SPL, CAT>);
   const int grid = occupancy_grid<double, R>(k, bd, lds, c->n_cu, a.n_jobs);
-  hipLaunchKernelGGL((marginal_mw_kernel<R, MAXBD, kSPL, CAT>), dim3(grid), dim3(bd), lds, c->stream, a);
+  hipLaunchKernelGGL((marginal_mw_kernel<R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->stream, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
 
 template <int R, bool CAT = false>
-int launch_marg(pp_ctx *c, OptArgs a, int bd) {
-  if (bd <= 512) return launch_marg_b<R, 512, CAT>(c, a, bd);
-  if (bd <= 768) return launch_marg_b<R, 768, CAT>(c, a, bd);
-  return launch_marg_b<R, 1024, CAT>(c, a, bd);
+int launch_marg(pp_ctx *c, OptArgs a, int b) {
+  const int spl = spl_for(true, b), bd = block_for(b, spl);
+  if (spl == 1) {
+    if (bd <= 512) return launch_marg_b<R, 512, 1, CAT>(c, a, bd);
+    if (bd <= 768) return launch_marg_b<R, 768, 1, CAT>(c, a, bd);
+    return launch_marg_b<R, 1024, 1, CAT>(c, a, bd);
+  }
+  if (bd <= 768) return launch_marg_b<R, 768, 2, CAT>(c, a, bd);
+  return launch_marg_b<R, 1024, 2, CAT>(c, a, bd);
 }
 
 // 20-state stage 2: one master + 7 worker waves per query, pairs per lane as
@@ -659,7 +694,7 @@ int upload_prior(pp_ctx *c, OptArgs *o) {
   return PP_OK;
 }
 
-int dispatch_marg(pp_ctx *c, OptArgs a, int bd) {
+int dispatch_marg(pp_ctx *c, OptArgs a, int b) {
   a.site_cat = site_cat_ptr(c);
   if (c->model.S == 20) {
     if (c->model.R == 4) return launch_marg_prot<4>(c, a);
@@ -667,15 +702,15 @@ int dispatch_marg(pp_ctx *c, OptArgs a, int bd) {
     set_error("n_rates must be 1 or 4 on the GPU path");
     return PP_EINVAL;
   }
-  if (c->model.n_cats > 0) return launch_marg<1, true>(c, a, bd);
-  if (c->model.R == 4) return launch_marg<4>(c, a, bd);
-  if (c->model.R == 1) return launch_marg<1>(c, a, bd);
+  if (c->model.n_cats > 0) return launch_marg<1, true>(c, a, b);
+  if (c->model.R == 4) return launch_marg<4>(c, a, b);
+  if (c->model.R == 1) return launch_marg<1>(c, a, b);
   set_error("n_rates must be 1 or 4 on the GPU path");
   return PP_EINVAL;
 }
 
 template <typename Real>
-int dispatch_pitch(pp_ctx *c, PitchArgs a, int bd) {
+int dispatch_pitch(pp_ctx *c, PitchArgs a, int b) {
   a.site_cat = site_cat_ptr(c);
   if (c->model.S == 20) {
     if (c->model.R == 4) return launch_pitch_prot<Real, 4>(c, a);
@@ -683,14 +718,14 @@ int dispatch_pitch(pp_ctx *c, PitchArgs a, int bd) {
     set_error("n_rates must be 1 or 4 on the GPU path");
     return PP_EINVAL;
   }
-  if (c->model.n_cats > 0) return launch_pitch<Real, 1, true>(c, a, bd);
-  if (c->model.R == 4) return launch_pitch<Real, 4>(c, a, bd);
-  if (c->model.R == 1) return launch_pitch<Real, 1>(c, a, bd);
+  if (c->model.n_cats > 0) return launch_pitch<Real, 1, true>(c, a, b);
+  if (c->model.R == 4) return launch_pitch<Real, 4>(c, a, b);
+  if (c->model.R == 1) return launch_pitch<Real, 1>(c, a, b);
   set_error("n_rates must be 1 or 4 on the GPU path");
   return PP_EINVAL;
 }
 template <typename Real>
-int dispatch_opt(pp_ctx *c, OptArgs a, int bd) {
+int dispatch_opt(pp_ctx *c, OptArgs a, int b) {
   a.site_cat = site_cat_ptr(c);
   if (c->model.S == 20) {
     if (c->model.R == 4) return launch_opt_prot<Real, 4>(c, a);
@@ -698,9 +733,9 @@ int dispatch_opt(pp_ctx *c, OptArgs a, int bd) {
     set_error("n_rates must be 1 or 4 on the GPU path");
     return PP_EINVAL;
   }
-  if (c->model.n_cats > 0) return launch_opt<Real, 1, true>(c, a, bd);
-  if (c->model.R == 4) return launch_opt<Real, 4>(c, a, bd);
-  if (c->model.R == 1) return launch_opt<Real, 1>(c, a, bd);
+  if (c->model.n_cats > 0) return launch_opt<Real, 1, true>(c, a, b);
+  if (c->model.R == 4) return launch_opt<Real, 4>(c, a, b);
+  if (c->model.R == 1) return launch_opt<Real, 1>(c, a, b);
   set_error("n_rates must be 1 or 4 on the GPU path");
   return PP_EINVAL;
 }
@@ -785,7 +820,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   ENSURE(c->b_kept, (size_t)chunk * K, &d_kept);
   ENSURE(c->b_nkeep, chunk, &d_nkeep);
   ENSURE(c->b_qlist, chunk, &d_qlist);
-  ENSURE(c->b_counter, 64, &d_counter);
+  ENSURE(c->b_counter, 128, &d_counter);
   ENSURE(c->b_stats, 6, &d_stats);
   HIPCHECK(hipMemsetAsync(d_stats, 0, 6 * sizeof(unsigned long long), c->stream));
   HIPCHECK(hipMemcpyAsync(d_status, c->qstat.data(), (size_t)Q * 4, hipMemcpyHostToDevice, c->stream));
@@ -836,7 +871,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     boff[kBuckets] = (int)flat.size();
     if (!flat.empty())
       HIPCHECK(hipMemcpyAsync(d_qlist, flat.data(), flat.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(hipMemsetAsync(d_counter, 0, 64 * 4, c->stream));
+    HIPCHECK(hipMemsetAsync(d_counter, 0, 128 * 4, c->stream));
     HIPCHECK(hipEventRecord(ev1b, c->stream));
     const uint8_t *codes = reinterpret_cast<const uint8_t *>(c->b_codes.p) + (size_t)q0 * N;
     int *status = d_status + q0;
@@ -869,7 +904,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
       p.status = status;
       p.n_calls = d_stats;
       p.n_pitched = d_stats + 1;
-      st = s2_fp32 ? dispatch_pitch<float>(c, p, block_of_bucket(b)) : dispatch_pitch<double>(c, p, block_of_bucket(b));
+      st = s2_fp32 ? dispatch_pitch<float>(c, p, b) : dispatch_pitch<double>(c, p, b);
       if (st) return st;
       launches++;
       pitch_launches++;
@@ -899,7 +934,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
       o.n_jobs = n * K;
       o.K = K;
       o.qlist = d_qlist + boff[b];
-      o.counter = d_counter + 16 + b;
+      o.counter = d_counter + 32 + b;
       o.darr = c->b_d64.p;
       o.parr = c->b_p64.p;
       o.darr_e = reinterpret_cast<const int *>(c->b_de.p);
@@ -912,7 +947,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
       o.n_keep = d_nkeep;
       o.status = status;
       o.n_calls = d_stats;
-      st = dispatch_opt<double>(c, o, block_of_bucket(b));
+      st = dispatch_opt<double>(c, o, b);
       if (st) return st;
       launches++;
     }
@@ -929,7 +964,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
         o.n_jobs = n * K;
         o.K = K;
         o.qlist = d_qlist + boff[b];
-        o.counter = d_counter + 32 + b;
+        o.counter = d_counter + 64 + b;
         o.darr = c->b_d64.p;
         o.parr = c->b_p64.p;
         o.darr_e = reinterpret_cast<const int *>(c->b_de.p);
@@ -941,7 +976,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
         o.n_keep = d_nkeep;
         o.status = status;
         o.n_calls = d_stats;
-        st = dispatch_marg(c, o, block_of_bucket(b));
+        st = dispatch_marg(c, o, b);
         if (st) return st;
         launches++;
       }
@@ -1338,13 +1373,13 @@ int pp_optimize_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
          *d_di = s.up<double>(nullptr, n_jobs);
   int *d_calls = s.up<int>(nullptr, n_jobs), *d_st = s.up<int>(nullptr, n_jobs);
   unsigned long long *d_stats = s.up<unsigned long long>(nullptr, 4);
-  int *d_counter = s.up<int>(nullptr, 16);
+  int *d_counter = s.up<int>(nullptr, 32);
   CHECK(d_stats && d_counter, PP_ENOMEM, "device allocation failed");
   std::vector<int> init_st(n_jobs, PP_Q_OK);
   for (int j = 0; j < n_jobs; j++)
     if (c->qstat[jq[j]] != PP_Q_OK) init_st[j] = 100 + c->qstat[jq[j]];
   HIPCHECK(hipMemcpy(d_st, init_st.data(), n_jobs * 4, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemset(d_counter, 0, 64));
+  HIPCHECK(hipMemset(d_counter, 0, 128));
   HIPCHECK(hipMemset(d_stats, 0, 32));
   std::vector<std::vector<int>> buckets(kBuckets);
   for (int j = 0; j < n_jobs; j++)
@@ -1389,7 +1424,7 @@ int pp_optimize_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
     o.out_calls = d_calls;
     o.out_status = d_st;
     o.n_calls = d_stats;
-    st = fp32 ? dispatch_opt<float>(c, o, block_of_bucket(b)) : dispatch_opt<double>(c, o, block_of_bucket(b));
+    st = fp32 ? dispatch_opt<float>(c, o, b) : dispatch_opt<double>(c, o, b);
     if (st) return st;
   }
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1425,14 +1460,14 @@ int pp_marginal_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
   double *d_m = s.up<double>(nullptr, n_jobs);
   int *d_calls = s.up<int>(nullptr, n_jobs), *d_st = s.up<int>(nullptr, n_jobs);
   unsigned long long *d_stats = s.up<unsigned long long>(nullptr, 4);
-  int *d_counter = s.up<int>(nullptr, 16);
+  int *d_counter = s.up<int>(nullptr, 32);
   CHECK(d_jq && d_loc && d_p0 && d_d0 && d_m && d_calls && d_st && d_stats && d_counter, PP_ENOMEM,
         "device allocation failed");
   std::vector<int> init_st(n_jobs, PP_Q_OK);
   for (int j = 0; j < n_jobs; j++)
     if (c->qstat[jq[j]] != PP_Q_OK) init_st[j] = 100 + c->qstat[jq[j]];
   HIPCHECK(hipMemcpy(d_st, init_st.data(), n_jobs * 4, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemset(d_counter, 0, 64));
+  HIPCHECK(hipMemset(d_counter, 0, 128));
   HIPCHECK(hipMemset(d_stats, 0, 32));
   std::vector<std::vector<int>> buckets(kBuckets);
   for (int j = 0; j < n_jobs; j++)
@@ -1472,7 +1507,7 @@ int pp_marginal_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
     o.out_calls = d_calls;
     o.out_status = d_st;
     o.n_calls = d_stats;
-    st = dispatch_marg(c, o, block_of_bucket(b));
+    st = dispatch_marg(c, o, b);
     if (st) return st;
   }
   HIPCHECK(hipStreamSynchronize(c->stream));

@@ -237,8 +237,19 @@ __device__ __forceinline__ void pitch_loop(const PitchArgs &a, Master &m, uint8_
   m.command(CMD_EXIT);
 }
 
+// Waves per SIMD the 4-state stage-2 kernels are built for.  Latency, not
+// issue, bounds a probe, so occupancy pays: with one FP64 site or two FP32
+// sites per worker lane the rows fit the 128 VGPRs of 4 waves per SIMD, and
+// the master wave's control state spills only outside the per-probe loop.
+// Two FP64 sites per lane (the longest queries) keep the launch bound's budget.
+template <typename Real, int SPL>
+struct MWOcc {
+  static constexpr int waves = (sizeof(Real) == 8 && SPL == 2) ? 1 : 4;
+};
+
 template <typename Real, int R, int MAXBD, int SPL, bool CAT>
-__global__ __launch_bounds__(MAXBD) void pitch_mw_kernel(PitchArgs a) {
+__global__ __launch_bounds__(MAXBD) __attribute__((amdgpu_waves_per_eu(MWOcc<Real, SPL>::waves)))
+void pitch_mw_kernel(PitchArgs a) {
   extern __shared__ __align__(16) char smem[];
   const MWLayout<Real, R> lay = mw_layout<Real, R>(a.model);
   MWShared *sh = reinterpret_cast<MWShared *>(smem);
@@ -406,7 +417,8 @@ __device__ __forceinline__ void marginal_loop(const OptArgs &a, Master &m) {
 }
 
 template <typename Real, int R, int MAXBD, int SPL, bool CAT>
-__global__ __launch_bounds__(MAXBD) void optimize_mw_kernel(OptArgs a) {
+__global__ __launch_bounds__(MAXBD) __attribute__((amdgpu_waves_per_eu(MWOcc<Real, SPL>::waves)))
+void optimize_mw_kernel(OptArgs a) {
   extern __shared__ __align__(16) char smem[];
   const MWLayout<Real, R> lay = mw_layout<Real, R>(a.model);
   MWShared *sh = reinterpret_cast<MWShared *>(smem);
@@ -441,7 +453,8 @@ __global__ __launch_bounds__(512) void optimize_prot_kernel(OptArgs a) {
 }
 
 template <int R, int MAXBD, int SPL, bool CAT>
-__global__ __launch_bounds__(MAXBD) void marginal_mw_kernel(OptArgs a) {
+__global__ __launch_bounds__(MAXBD) __attribute__((amdgpu_waves_per_eu(MWOcc<double, SPL>::waves)))
+void marginal_mw_kernel(OptArgs a) {
   extern __shared__ __align__(16) char smem[];
   const MWLayout<double, R> lay = mw_layout<double, R>(a.model);
   mw_copy_model(reinterpret_cast<MWShared *>(smem), a.model);
