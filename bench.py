@@ -225,7 +225,11 @@ def main():
                     "note": "gathered table bytes; most are served by L1/L2"}
     else:
         # the 20-state form runs stage 2 in FP64 in both forms
-        name = ("pitch_mw_kernel<%s,4,512,2>" % real if S == 4
+        # 4 states: one FP64 site or two FP32 sites per worker lane; the FP64
+        # kernel runs in the 512- and 768-lane launch classes (masked lengths
+        # up to 448 and beyond), named together as MAXBD = *
+        name = ("pitch_mw_kernel<double,4,*,1,false>" if S == 4 and real == "double"
+                else "pitch_mw_kernel<float,4,512,2,false>" if S == 4
                 else "pitch_prot_kernel<double,4>")
         achieved = st_main["pitch_u2_tflops"]
         roofline = {"bound": "mfma",
