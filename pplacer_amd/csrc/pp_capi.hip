@@ -46,6 +46,7 @@ namespace {
 constexpr int kTQ = 16;           // queries per stage-1 tile
 constexpr int kChunk = 32768;     // queries per device batch
 constexpr int kBuckets = 30;      // stage-2 masked-length classes of 64 sites
+constexpr int kBucketStreams = 4; // streams running a stage-2 phase's buckets concurrently
 
 struct DBuf {
   void *p = nullptr;
@@ -82,7 +83,10 @@ struct pp_ctx {
   int device = 0;
   int n_cu = 256;
   size_t max_lds = 65536;  // bytes of LDS one workgroup may use
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;       // main stream: stage 2 and everything else
+  hipStream_t s1 = nullptr;           // stage 1 of the next chunk
+  hipStream_t bs[kBucketStreams] = {};  // concurrent stage-2 buckets
+  hipStream_t cur = nullptr;          // the stream the launch helpers use
   HostModel model;
   DBuf b_model;
   DevModel *d_model = nullptr;
@@ -440,39 +444,42 @@ int build_code_logs(pp_ctx *c, double start_pend) {
   return PP_OK;
 }
 
-// Stage-1 scores of queries [q0, q0 + nq) (sorted order) into scores[nq][E];
-// build_code_logs<Real> must have run for the same start_pend.
-template <typename Real>
-int launch_stage1(pp_ctx *c, int q0, int nq, double *scores) {
-  const int N = c->N, R = c->model.R;
+// Stage-1 tiles of queries [q0, q0 + nq) (sorted order): kTQ queries per
+// tile (chunk-relative indices, -1 for empty or failed slots) and the span of
+// columns their informative characters cover, appended to lo/hi/q.
+void stage1_tiles(const pp_ctx *c, int q0, int nq, std::vector<int> &lo_v, std::vector<int> &hi_v,
+                  std::vector<int> &q_v) {
+  const int N = c->N;
   const int n_tiles = (nq + kTQ - 1) / kTQ;
-  std::vector<int> tlo(n_tiles), thi(n_tiles), tq(n_tiles * kTQ, -1);
   for (int t = 0; t < n_tiles; t++) {
     int lo = N, hi = 0;
     for (int j = 0; j < kTQ; j++) {
-      int l = t * kTQ + j;
-      if (l >= nq) break;
-      int q = q0 + l;
-      if (c->qstat[q] != PP_Q_OK) continue;
-      tq[t * kTQ + j] = l;
+      const int l = t * kTQ + j;
+      const int q = q0 + l;
+      const bool ok = l < nq && c->qstat[q] == PP_Q_OK;
+      q_v.push_back(ok ? l : -1);
+      if (!ok) continue;
       lo = std::min(lo, c->qlo[q]);
       hi = std::max(hi, c->qhi[q]);
     }
     if (hi < lo) { lo = 0; hi = 0; }
-    tlo[t] = lo;
-    thi[t] = hi;
+    lo_v.push_back(lo);
+    hi_v.push_back(hi);
   }
-  int *d_tlo, *d_thi, *d_tq;
+}
+
+// Stage-1 scores of queries [q0, q0 + nq) into scores[nq][E] from tiles
+// already on the device; build_code_logs<Real> must have run for the same
+// start_pend.  Launches on c->cur.
+template <typename Real>
+int launch_stage1_tiles(pp_ctx *c, int q0, int nq, double *scores, const int *d_tlo,
+                        const int *d_thi, const int *d_tq) {
+  const int N = c->N, R = c->model.R;
+  const int n_tiles = (nq + kTQ - 1) / kTQ;
   uint8_t *d_tc;
-  ENSURE(c->b_tile_lo, n_tiles, &d_tlo);
-  ENSURE(c->b_tile_hi, n_tiles, &d_thi);
-  ENSURE(c->b_tile_q, n_tiles * kTQ, &d_tq);
   ENSURE(c->b_tcodes, (size_t)n_tiles * N * kTQ, &d_tc);
-  HIPCHECK(hipMemcpyAsync(d_tlo, tlo.data(), n_tiles * 4, hipMemcpyHostToDevice, c->stream));
-  HIPCHECK(hipMemcpyAsync(d_thi, thi.data(), n_tiles * 4, hipMemcpyHostToDevice, c->stream));
-  HIPCHECK(hipMemcpyAsync(d_tq, tq.data(), n_tiles * kTQ * 4, hipMemcpyHostToDevice, c->stream));
   const uint8_t *codes = reinterpret_cast<const uint8_t *>(c->b_codes.p) + (size_t)q0 * N;
-  hipLaunchKernelGGL(tile_codes_kernel, dim3(2048), dim3(256), 0, c->stream, codes, d_tq, n_tiles,
+  hipLaunchKernelGGL(tile_codes_kernel, dim3(2048), dim3(256), 0, c->cur, codes, d_tq, n_tiles,
                      N, kTQ, (uint8_t)c->model.n_inf, d_tc);
   Stage1Args a;
   a.lt = c->b_lt.p;
@@ -489,9 +496,27 @@ int launch_stage1(pp_ctx *c, int q0, int nq, double *scores) {
   a.log_R = std::log((double)(float)R);
   a.scores = scores;
   dim3 grid(c->Epad / 256, n_tiles);
-  hipLaunchKernelGGL((stage1_kernel<Real, kTQ>), grid, dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL((stage1_kernel<Real, kTQ>), grid, dim3(256), 0, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
+}
+
+// Stage-1 scores of queries [q0, q0 + nq) into scores[nq][E], tiles uploaded
+// here (synchronous copies).
+template <typename Real>
+int launch_stage1(pp_ctx *c, int q0, int nq, double *scores) {
+  std::vector<int> lo, hi, tq;
+  stage1_tiles(c, q0, nq, lo, hi, tq);
+  const int n_tiles = (int)lo.size();
+  int *d_tlo, *d_thi, *d_tq;
+  ENSURE(c->b_tile_lo, std::max(n_tiles, 1), &d_tlo);
+  ENSURE(c->b_tile_hi, std::max(n_tiles, 1), &d_thi);
+  ENSURE(c->b_tile_q, std::max(n_tiles, 1) * kTQ, &d_tq);
+  HIPCHECK(hipStreamSynchronize(c->cur));
+  HIPCHECK(hipMemcpy(d_tlo, lo.data(), n_tiles * 4, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_thi, hi.data(), n_tiles * 4, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_tq, tq.data(), (size_t)n_tiles * kTQ * 4, hipMemcpyHostToDevice));
+  return launch_stage1_tiles<Real>(c, q0, nq, scores, d_tlo, d_thi, d_tq);
 }
 
 template <typename Real, int R>
@@ -540,7 +565,7 @@ int launch_pitch_b(pp_ctx *c, PitchArgs a, int bd) {
   const size_t lds = stage2_lds<Real, R>(c, bd, c->E, SPL);
   const void *k = reinterpret_cast<const void *>(&pitch_mw_kernel<Real, R, MAXBD, SPL, CAT>);
   const int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.nq);
-  hipLaunchKernelGGL((pitch_mw_kernel<Real, R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->stream, a);
+  hipLaunchKernelGGL((pitch_mw_kernel<Real, R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
@@ -550,7 +575,7 @@ int launch_opt_b(pp_ctx *c, OptArgs a, int bd) {
   const size_t lds = stage2_lds<Real, R>(c, bd, 0, SPL);
   const void *k = reinterpret_cast<const void *>(&optimize_mw_kernel<Real, R, MAXBD, SPL, CAT>);
   const int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.n_jobs);
-  hipLaunchKernelGGL((optimize_mw_kernel<Real, R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->stream, a);
+  hipLaunchKernelGGL((optimize_mw_kernel<Real, R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
@@ -596,7 +621,7 @@ int launch_marg_b(pp_ctx *c, OptArgs a, int bd) {
   const size_t lds = stage2_lds<double, R>(c, bd, 0, SPL);
   const void *k = reinterpret_cast<const void *>(&marginal_mw_kernel<R, MAXBD, SPL, CAT>);
   const int grid = occupancy_grid<double, R>(k, bd, lds, c->n_cu, a.n_jobs);
-  hipLaunchKernelGGL((marginal_mw_kernel<R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->stream, a);
+  hipLaunchKernelGGL((marginal_mw_kernel<R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
@@ -643,7 +668,7 @@ int launch_pitch_prot(pp_ctx *c, PitchArgs a) {
   a.cap_pairs = prot_cap_pairs(c);
   int st = prot_scratch<Real>(c, k, lds, a.cap_pairs, &a.scratch);
   if (st) return st;
-  hipLaunchKernelGGL((pitch_prot_kernel<Real, R>), dim3(grid), dim3(512), lds, c->stream, a);
+  hipLaunchKernelGGL((pitch_prot_kernel<Real, R>), dim3(grid), dim3(512), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
@@ -656,7 +681,7 @@ int launch_opt_prot(pp_ctx *c, OptArgs a) {
   a.cap_pairs = prot_cap_pairs(c);
   int st = prot_scratch<Real>(c, k, lds, a.cap_pairs, &a.scratch);
   if (st) return st;
-  hipLaunchKernelGGL((optimize_prot_kernel<Real, R>), dim3(grid), dim3(512), lds, c->stream, a);
+  hipLaunchKernelGGL((optimize_prot_kernel<Real, R>), dim3(grid), dim3(512), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
@@ -669,7 +694,7 @@ int launch_marg_prot(pp_ctx *c, OptArgs a) {
   a.cap_pairs = prot_cap_pairs(c);
   int st = prot_scratch<double>(c, k, lds, a.cap_pairs, &a.scratch);
   if (st) return st;
-  hipLaunchKernelGGL((marginal_prot_kernel<R>), dim3(grid), dim3(512), lds, c->stream, a);
+  hipLaunchKernelGGL((marginal_prot_kernel<R>), dim3(grid), dim3(512), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
@@ -792,6 +817,34 @@ int queries_upload(pp_ctx *c, int n_queries, const char *queries) {
   return PP_OK;
 }
 
+// Events of one placement pass, destroyed with it.
+struct EventPool {
+  std::vector<hipEvent_t> ev;
+  hipEvent_t get() {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ev.push_back(e);
+    return e;
+  }
+  ~EventPool() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+// Restores the context's launch stream on every exit path.
+struct LaunchStream {
+  pp_ctx *c;
+  LaunchStream(pp_ctx *c_, hipStream_t s) : c(c_) { c->cur = s; }
+  void set(hipStream_t s) { c->cur = s; }
+  ~LaunchStream() { c->cur = c->stream; }
+};
+
+// A placement pass over the resident queries, in chunks.  Stage 1 of every
+// chunk runs on its own stream into one of two score buffers, so it overlaps
+// stage 2 of the previous chunk; stage 2 (baseball, keep, refinement, -p,
+// finalize) runs chunk after chunk on the main stream; in the 4-state FP32
+// form the buckets of each phase run concurrently on kBucketStreams streams
+// (the tails of their work queues overlap), joined before the next phase.
 int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats) {
   CHECK(prefs, PP_EINVAL, "prefs is null");
   CHECK(precision == PP_FP64 || precision == PP_FP32, PP_EINVAL, "precision must be PP_FP64 or PP_FP32");
@@ -805,6 +858,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   else { t_max_pitches = prefs->max_pitches; t_max_strikes = prefs->max_strikes; }
   const int cap = (int)std::min<long long>(E, std::max<long long>(1, t_max_pitches + 1));
   const int chunk = std::max(1, std::min(Q, (int)std::min<long long>(kChunk, (1LL << 31) / ((long long)E * 8 + 1))));
+  const int n_chunks = Q > 0 ? (Q + chunk - 1) / chunk : 0;
   pp_placement *d_out;
   int *d_nout, *d_status;
   ENSURE(c->b_out, (size_t)std::max(Q, 1) * K, &d_out);
@@ -814,18 +868,17 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   Res *d_res, *d_kept;
   int *d_nres, *d_nkeep, *d_qlist, *d_counter;
   unsigned long long *d_stats;
-  ENSURE(c->b_scores, (size_t)chunk * E, &d_scores);
+  const int n_sbuf = n_chunks > 1 ? 2 : 1;
+  ENSURE(c->b_scores, (size_t)n_sbuf * chunk * E, &d_scores);
   ENSURE(c->b_res, (size_t)chunk * cap, &d_res);
   ENSURE(c->b_nres, chunk, &d_nres);
   ENSURE(c->b_kept, (size_t)chunk * K, &d_kept);
   ENSURE(c->b_nkeep, chunk, &d_nkeep);
-  ENSURE(c->b_qlist, chunk, &d_qlist);
   ENSURE(c->b_counter, 128, &d_counter);
   ENSURE(c->b_stats, 6, &d_stats);
   HIPCHECK(hipMemsetAsync(d_stats, 0, 6 * sizeof(unsigned long long), c->stream));
   HIPCHECK(hipMemcpyAsync(d_status, c->qstat.data(), (size_t)Q * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemsetAsync(d_nout, 0, (size_t)std::max(Q, 1) * 4, c->stream));
-  double t_s1 = 0, t_pitch = 0, t_ref = 0, t_pp = 0;
   long long launches = 0, pitch_launches = 0;
   OptArgs marg_proto{};
   std::memset(&marg_proto, 0, sizeof(marg_proto));
@@ -833,57 +886,139 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     st = upload_prior(c, &marg_proto);
     if (st) return st;
   }
-  hipEvent_t ev0, ev1, ev1b, ev2, ev3, evp0, evp1;
-  HIPCHECK(hipEventCreate(&ev0));
-  HIPCHECK(hipEventCreate(&ev1));
-  HIPCHECK(hipEventCreate(&ev1b));
-  HIPCHECK(hipEventCreate(&ev2));
-  HIPCHECK(hipEventCreate(&ev3));
-  HIPCHECK(hipEventCreate(&evp0));
-  HIPCHECK(hipEventCreate(&evp1));
+  EventPool pool;
+  LaunchStream ls(c, c->stream);
   const bool fp32 = precision == PP_FP32;
   const bool s2_fp32 = fp32 && stage2_fp32_ok(c);
-  float t_table = 0;
-  HIPCHECK(hipEventRecord(ev0, c->stream));
+  // Concurrent buckets in the FP32 form (100,000 queries: 323 -> 274 ms per
+  // pass).  In the FP64 form they slow each other's spilling FP64 kernels
+  // (383 -> 386 ms, against 374 with stage-1 overlap alone), and the 20-state
+  // kernels share one scratch buffer.
+  const bool fork = c->model.S == 4 && s2_fp32;
+  hipEvent_t e_t0 = pool.get(), e_t1 = pool.get();
+  CHECK(e_t0 && e_t1, PP_EDEVICE, "hipEventCreate failed");
+  HIPCHECK(hipEventRecord(e_t0, c->stream));
   st = fp32 ? build_code_logs<float>(c, prefs->start_pend) : build_code_logs<double>(c, prefs->start_pend);
   if (st) return st;
   launches++;
-  HIPCHECK(hipEventRecord(ev1, c->stream));
-  HIPCHECK(hipEventSynchronize(ev1));
-  HIPCHECK(hipEventElapsedTime(&t_table, ev0, ev1));
-  for (int q0 = 0; q0 < Q; q0 += chunk) {
-    const int nq = std::min(chunk, Q - q0);
-    HIPCHECK(hipEventRecord(ev0, c->stream));
-    st = fp32 ? launch_stage1<float>(c, q0, nq, d_scores) : launch_stage1<double>(c, q0, nq, d_scores);
-    if (st) return st;
-    launches += 2;
-    HIPCHECK(hipEventRecord(ev1, c->stream));
-    // bucket queries of the chunk by masked length
+  HIPCHECK(hipEventRecord(e_t1, c->stream));
+  HIPCHECK(hipStreamWaitEvent(c->s1, e_t1, 0));  // stage 1 reads the table
+  // Host-side plans of every chunk, uploaded once with synchronous copies so
+  // that no asynchronous copy reads host memory: stage-1 tiles, and the
+  // chunk's queries bucketed by masked length (qlist + per-bucket offsets).
+  // qlist[q0 .. q0 + nq) holds chunk q0's queries by bucket, boffs[ci] the
+  // bucket offsets within it
+  std::vector<int> t_lo, t_hi, t_q, t_off(n_chunks + 1, 0), qlist_all(std::max(Q, 1), -1);
+  std::vector<std::vector<int>> boffs(n_chunks, std::vector<int>(kBuckets + 1, 0));
+  for (int ci = 0; ci < n_chunks; ci++) {
+    const int q0 = ci * chunk, nq = std::min(chunk, Q - q0);
+    t_off[ci] = (int)t_lo.size();
+    stage1_tiles(c, q0, nq, t_lo, t_hi, t_q);
     std::vector<std::vector<int>> buckets(kBuckets);
     for (int l = 0; l < nq; l++)
       if (c->qstat[q0 + l] == PP_Q_OK) buckets[bucket_of(c->qM[q0 + l])].push_back(l);
-    std::vector<int> flat;
-    std::vector<int> boff(kBuckets + 1, 0);
+    std::vector<int> &boff = boffs[ci];
+    int n = 0;
     for (int b = 0; b < kBuckets; b++) {
-      boff[b] = (int)flat.size();
-      flat.insert(flat.end(), buckets[b].begin(), buckets[b].end());
+      boff[b] = n;
+      for (int l : buckets[b]) qlist_all[(size_t)q0 + n++] = l;
     }
-    boff[kBuckets] = (int)flat.size();
-    if (!flat.empty())
-      HIPCHECK(hipMemcpyAsync(d_qlist, flat.data(), flat.size() * 4, hipMemcpyHostToDevice, c->stream));
+    boff[kBuckets] = n;
+  }
+  t_off[n_chunks] = (int)t_lo.size();
+  int *d_tlo, *d_thi, *d_tq;
+  const int n_tiles_all = std::max(1, (int)t_lo.size());
+  ENSURE(c->b_tile_lo, n_tiles_all, &d_tlo);
+  ENSURE(c->b_tile_hi, n_tiles_all, &d_thi);
+  ENSURE(c->b_tile_q, (size_t)n_tiles_all * kTQ, &d_tq);
+  uint8_t *d_tc_prealloc;
+  ENSURE(c->b_tcodes, (size_t)((std::min(chunk, std::max(Q, 1)) + kTQ - 1) / kTQ) * N * kTQ, &d_tc_prealloc);
+  (void)d_tc_prealloc;
+  ENSURE(c->b_qlist, std::max(Q, 1), &d_qlist);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  if (!t_lo.empty()) {
+    HIPCHECK(hipMemcpy(d_tlo, t_lo.data(), t_lo.size() * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(d_thi, t_hi.data(), t_hi.size() * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(d_tq, t_q.data(), t_q.size() * 4, hipMemcpyHostToDevice));
+  }
+  if (Q > 0) HIPCHECK(hipMemcpy(d_qlist, qlist_all.data(), (size_t)Q * 4, hipMemcpyHostToDevice));
+  // per-chunk intervals, read after the pass: stage 1 (s1 stream), pitch,
+  // keep + refine, -p, finalize (main stream)
+  struct Marks { hipEvent_t s1a, s1b, p0, p1, r1, m1, f1; };
+  std::vector<Marks> marks(n_chunks);
+  std::vector<hipEvent_t> scores_free(n_sbuf, nullptr);  // last pitch phase reading each buffer
+  // Runs launch(b) for every non-empty bucket, concurrently on the bucket
+  // streams when forking, and joins them back into the main stream.
+  auto phase = [&](const std::vector<int> &boff, auto &&launch) -> int {
+    if (!fork) {
+      ls.set(c->stream);
+      for (int b = 0; b < kBuckets; b++)
+        if (boff[b + 1] > boff[b]) {
+          const int r = launch(b);
+          if (r) return r;
+        }
+      return PP_OK;
+    }
+    hipEvent_t ef = pool.get();
+    CHECK(ef, PP_EDEVICE, "hipEventCreate failed");
+    HIPCHECK(hipEventRecord(ef, c->stream));
+    bool used[kBucketStreams] = {};
+    int k = 0;
+    for (int b = 0; b < kBuckets; b++) {
+      if (boff[b + 1] == boff[b]) continue;
+      const int si = k++ % kBucketStreams;
+      if (!used[si]) {
+        HIPCHECK(hipStreamWaitEvent(c->bs[si], ef, 0));
+        used[si] = true;
+      }
+      ls.set(c->bs[si]);
+      const int r = launch(b);
+      if (r) return r;
+    }
+    ls.set(c->stream);
+    for (int si = 0; si < kBucketStreams; si++) {
+      if (!used[si]) continue;
+      hipEvent_t ej = pool.get();
+      CHECK(ej, PP_EDEVICE, "hipEventCreate failed");
+      HIPCHECK(hipEventRecord(ej, c->bs[si]));
+      HIPCHECK(hipStreamWaitEvent(c->stream, ej, 0));
+    }
+    return PP_OK;
+  };
+  for (int ci = 0; ci < n_chunks; ci++) {
+    const int q0 = ci * chunk, nq = std::min(chunk, Q - q0);
+    double *scores = d_scores + (size_t)(ci % n_sbuf) * chunk * E;
+    Marks &mk = marks[ci];
+    for (hipEvent_t *e : {&mk.s1a, &mk.s1b, &mk.p0, &mk.p1, &mk.r1, &mk.m1, &mk.f1}) {
+      *e = pool.get();
+      CHECK(*e, PP_EDEVICE, "hipEventCreate failed");
+    }
+    // stage 1 on its stream, once the baseball phase two chunks back has
+    // finished with this score buffer
+    if (scores_free[ci % n_sbuf]) HIPCHECK(hipStreamWaitEvent(c->s1, scores_free[ci % n_sbuf], 0));
+    ls.set(c->s1);
+    HIPCHECK(hipEventRecord(mk.s1a, c->s1));
+    const int to = t_off[ci];
+    st = fp32 ? launch_stage1_tiles<float>(c, q0, nq, scores, d_tlo + to, d_thi + to, d_tq + (size_t)to * kTQ)
+              : launch_stage1_tiles<double>(c, q0, nq, scores, d_tlo + to, d_thi + to, d_tq + (size_t)to * kTQ);
+    if (st) return st;
+    launches += 2;
+    HIPCHECK(hipEventRecord(mk.s1b, c->s1));
+    ls.set(c->stream);
+    const std::vector<int> &boff = boffs[ci];
+    int *qlist = d_qlist + q0;
     HIPCHECK(hipMemsetAsync(d_counter, 0, 128 * 4, c->stream));
-    HIPCHECK(hipEventRecord(ev1b, c->stream));
+    HIPCHECK(hipStreamWaitEvent(c->stream, mk.s1b, 0));
+    HIPCHECK(hipEventRecord(mk.p0, c->stream));
     const uint8_t *codes = reinterpret_cast<const uint8_t *>(c->b_codes.p) + (size_t)q0 * N;
     int *status = d_status + q0;
-    for (int b = 0; b < kBuckets; b++) {
-      const int n = boff[b + 1] - boff[b];
-      if (!n) continue;
+    st = phase(boff, [&](int b) -> int {
       PitchArgs p{};
       p.model = c->d_model;
       p.codes = codes;
-      p.scores = d_scores;
-      p.qlist = d_qlist + boff[b];
-      p.nq = n;
+      p.scores = scores;
+      p.qlist = qlist + boff[b];
+      p.nq = boff[b + 1] - boff[b];
       p.counter = d_counter + b;
       p.darr = s2_fp32 ? c->b_d32.p : c->b_d64.p;
       p.parr = s2_fp32 ? c->b_p32.p : c->b_p64.p;
@@ -904,12 +1039,13 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
       p.status = status;
       p.n_calls = d_stats;
       p.n_pitched = d_stats + 1;
-      st = s2_fp32 ? dispatch_pitch<float>(c, p, b) : dispatch_pitch<double>(c, p, b);
-      if (st) return st;
       launches++;
       pitch_launches++;
-    }
-    HIPCHECK(hipEventRecord(ev2, c->stream));
+      return s2_fp32 ? dispatch_pitch<float>(c, p, b) : dispatch_pitch<double>(c, p, b);
+    });
+    if (st) return st;
+    HIPCHECK(hipEventRecord(mk.p1, c->stream));
+    scores_free[ci % n_sbuf] = mk.p1;
     KeepArgs ka;
     ka.Q = nq;
     ka.cap = cap;
@@ -922,18 +1058,16 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     ka.n_keep = d_nkeep;
     hipLaunchKernelGGL(keep_kernel, dim3((nq + 255) / 256), dim3(256), 0, c->stream, ka);
     HIPCHECK(hipGetLastError());
-    for (int b = 0; b < kBuckets; b++) {
-      const int n = boff[b + 1] - boff[b];
-      if (!n) continue;
+    st = phase(boff, [&](int b) -> int {
       OptArgs o{};
       std::memset(&o, 0, sizeof(o));
       o.model = c->d_model;
       o.codes = codes;
       o.N = N;
       o.mode = 1;
-      o.n_jobs = n * K;
+      o.n_jobs = (boff[b + 1] - boff[b]) * K;
       o.K = K;
-      o.qlist = d_qlist + boff[b];
+      o.qlist = qlist + boff[b];
       o.counter = d_counter + 32 + b;
       o.darr = c->b_d64.p;
       o.parr = c->b_p64.p;
@@ -947,23 +1081,21 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
       o.n_keep = d_nkeep;
       o.status = status;
       o.n_calls = d_stats;
-      st = dispatch_opt<double>(c, o, b);
-      if (st) return st;
       launches++;
-    }
-    HIPCHECK(hipEventRecord(evp0, c->stream));
+      return dispatch_opt<double>(c, o, b);
+    });
+    if (st) return st;
+    HIPCHECK(hipEventRecord(mk.r1, c->stream));
     if (prefs->calc_pp) {  // core.ml:378-392, one job per kept placement
-      for (int b = 0; b < kBuckets; b++) {
-        const int n = boff[b + 1] - boff[b];
-        if (!n) continue;
+      st = phase(boff, [&](int b) -> int {
         OptArgs o = marg_proto;
         o.model = c->d_model;
         o.codes = codes;
         o.N = N;
         o.mode = 1;
-        o.n_jobs = n * K;
+        o.n_jobs = (boff[b + 1] - boff[b]) * K;
         o.K = K;
-        o.qlist = d_qlist + boff[b];
+        o.qlist = qlist + boff[b];
         o.counter = d_counter + 64 + b;
         o.darr = c->b_d64.p;
         o.parr = c->b_p64.p;
@@ -976,12 +1108,12 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
         o.n_keep = d_nkeep;
         o.status = status;
         o.n_calls = d_stats;
-        st = dispatch_marg(c, o, b);
-        if (st) return st;
         launches++;
-      }
+        return dispatch_marg(c, o, b);
+      });
+      if (st) return st;
     }
-    HIPCHECK(hipEventRecord(evp1, c->stream));
+    HIPCHECK(hipEventRecord(mk.m1, c->stream));
     FinalArgs fa;
     fa.Q = nq;
     fa.K = K;
@@ -994,20 +1126,25 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     hipLaunchKernelGGL(finalize_kernel, dim3((nq + 255) / 256), dim3(256), 0, c->stream, fa);
     HIPCHECK(hipGetLastError());
     launches += 2;
-    HIPCHECK(hipEventRecord(ev3, c->stream));
-    HIPCHECK(hipEventSynchronize(ev3));
+    HIPCHECK(hipEventRecord(mk.f1, c->stream));
+  }
+  HIPCHECK(hipStreamSynchronize(c->s1));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  float t_table = 0;
+  double t_s1 = 0, t_pitch = 0, t_ref = 0, t_pp = 0;
+  HIPCHECK(hipEventElapsedTime(&t_table, e_t0, e_t1));
+  for (const Marks &mk : marks) {
     float a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0;
-    HIPCHECK(hipEventElapsedTime(&a1, ev0, ev1));
-    HIPCHECK(hipEventElapsedTime(&a2, ev1b, ev2));
-    HIPCHECK(hipEventElapsedTime(&a3, ev2, evp0));
-    HIPCHECK(hipEventElapsedTime(&a4, evp0, evp1));
-    HIPCHECK(hipEventElapsedTime(&a5, evp1, ev3));
+    HIPCHECK(hipEventElapsedTime(&a1, mk.s1a, mk.s1b));
+    HIPCHECK(hipEventElapsedTime(&a2, mk.p0, mk.p1));
+    HIPCHECK(hipEventElapsedTime(&a3, mk.p1, mk.r1));
+    HIPCHECK(hipEventElapsedTime(&a4, mk.r1, mk.m1));
+    HIPCHECK(hipEventElapsedTime(&a5, mk.m1, mk.f1));
     t_s1 += a1;
     t_pitch += a2;
     t_ref += a3 + a5;
     t_pp += a4;
   }
-  for (hipEvent_t e : {ev0, ev1, ev1b, ev2, ev3, evp0, evp1}) (void)hipEventDestroy(e);
   if (stats) {
     unsigned long long hs[6];
     HIPCHECK(hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost));
@@ -1090,6 +1227,9 @@ int pp_ctx_create(int device, pp_ctx **out) {
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
     c->max_lds = (size_t)lds;
   HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHECK(hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking));
+  for (hipStream_t &b : c->bs) HIPCHECK(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
+  c->cur = c->stream;
   *out = c;
   return PP_OK;
 }
@@ -1098,6 +1238,9 @@ int pp_ctx_destroy(pp_ctx *c) {
   if (!c) return PP_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  if (c->s1) (void)hipStreamSynchronize(c->s1);
+  for (hipStream_t b : c->bs)
+    if (b) (void)hipStreamSynchronize(b);
   DBuf *bufs[] = {&c->b_model, &c->b_bl, &c->b_d64, &c->b_p64, &c->b_d32, &c->b_p32, &c->b_de,
                   &c->b_pe, &c->b_s64, &c->b_se, &c->b_de32, &c->b_pe32, &c->b_lt, &c->b_scratch, &c->b_cat, &c->b_prior, &c->b_codes, &c->b_qM, &c->b_tcodes,
                   &c->b_tile_lo, &c->b_tile_hi, &c->b_tile_q, &c->b_ttab, &c->b_scores, &c->b_res,
@@ -1106,6 +1249,9 @@ int pp_ctx_destroy(pp_ctx *c) {
   for (DBuf *b : bufs)
     if (b->p) (void)hipFree(b->p);
   (void)hipStreamDestroy(c->stream);
+  if (c->s1) (void)hipStreamDestroy(c->s1);
+  for (hipStream_t b : c->bs)
+    if (b) (void)hipStreamDestroy(b);
   delete c;
   return PP_OK;
 }
