@@ -47,6 +47,7 @@ constexpr int kTQ = 16;           // queries per stage-1 tile
 constexpr int kChunk = 32768;     // queries per device batch
 constexpr int kBuckets = 30;      // stage-2 masked-length classes of 64 sites
 constexpr int kBucketStreams = 4; // streams running a stage-2 phase's buckets concurrently
+constexpr int kProtBlock = 512;   // lanes of a 20-state stage-2 workgroup (master + workers)
 
 struct DBuf {
   void *p = nullptr;
@@ -644,7 +645,7 @@ int launch_marg(pp_ctx *c, OptArgs a, int b) {
 // workgroups never reallocate it while earlier launches may still run.
 template <typename Real>
 int prot_scratch(pp_ctx *c, const void *kern, size_t lds, int cap_pairs, void **out) {
-  const int grid_full = occupancy_grid<Real, 4>(kern, 512, lds, c->n_cu, INT_MAX);
+  const int grid_full = occupancy_grid<Real, 4>(kern, kProtBlock, lds, c->n_cu, INT_MAX);
   Real *p;
   ENSURE(c->b_scratch, ProtLayout::scratch_reals(cap_pairs) * (size_t)grid_full, &p);
   *out = p;
@@ -652,7 +653,7 @@ int prot_scratch(pp_ctx *c, const void *kern, size_t lds, int cap_pairs, void **
 }
 
 int prot_cap_pairs(const pp_ctx *c) {
-  constexpr int worker_lanes = 512 - 64;
+  constexpr int worker_lanes = kProtBlock - 64;
   int max_m = 1;
   for (int q = 0; q < c->Q; q++)
     if (c->qstat[q] == PP_Q_OK) max_m = std::max(max_m, c->qM[q]);
@@ -664,11 +665,11 @@ template <typename Real, int R>
 int launch_pitch_prot(pp_ctx *c, PitchArgs a) {
   const size_t lds = ProtLayout::bytes(c->N) + (size_t)c->E;
   const void *k = reinterpret_cast<const void *>(&pitch_prot_kernel<Real, R>);
-  const int grid = occupancy_grid<Real, R>(k, 512, lds, c->n_cu, a.nq);
+  const int grid = occupancy_grid<Real, R>(k, kProtBlock, lds, c->n_cu, a.nq);
   a.cap_pairs = prot_cap_pairs(c);
   int st = prot_scratch<Real>(c, k, lds, a.cap_pairs, &a.scratch);
   if (st) return st;
-  hipLaunchKernelGGL((pitch_prot_kernel<Real, R>), dim3(grid), dim3(512), lds, c->cur, a);
+  hipLaunchKernelGGL((pitch_prot_kernel<Real, R>), dim3(grid), dim3(kProtBlock), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
@@ -677,11 +678,11 @@ template <typename Real, int R>
 int launch_opt_prot(pp_ctx *c, OptArgs a) {
   const size_t lds = ProtLayout::bytes(c->N);
   const void *k = reinterpret_cast<const void *>(&optimize_prot_kernel<Real, R>);
-  const int grid = occupancy_grid<Real, R>(k, 512, lds, c->n_cu, a.n_jobs);
+  const int grid = occupancy_grid<Real, R>(k, kProtBlock, lds, c->n_cu, a.n_jobs);
   a.cap_pairs = prot_cap_pairs(c);
   int st = prot_scratch<Real>(c, k, lds, a.cap_pairs, &a.scratch);
   if (st) return st;
-  hipLaunchKernelGGL((optimize_prot_kernel<Real, R>), dim3(grid), dim3(512), lds, c->cur, a);
+  hipLaunchKernelGGL((optimize_prot_kernel<Real, R>), dim3(grid), dim3(kProtBlock), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
@@ -690,11 +691,11 @@ template <int R>
 int launch_marg_prot(pp_ctx *c, OptArgs a) {
   const size_t lds = ProtLayout::bytes(c->N);
   const void *k = reinterpret_cast<const void *>(&marginal_prot_kernel<R>);
-  const int grid = occupancy_grid<double, R>(k, 512, lds, c->n_cu, a.n_jobs);
+  const int grid = occupancy_grid<double, R>(k, kProtBlock, lds, c->n_cu, a.n_jobs);
   a.cap_pairs = prot_cap_pairs(c);
   int st = prot_scratch<double>(c, k, lds, a.cap_pairs, &a.scratch);
   if (st) return st;
-  hipLaunchKernelGGL((marginal_prot_kernel<R>), dim3(grid), dim3(512), lds, c->cur, a);
+  hipLaunchKernelGGL((marginal_prot_kernel<R>), dim3(grid), dim3(kProtBlock), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
