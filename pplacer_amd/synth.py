@@ -120,8 +120,38 @@ def _evolve_states(rng, states, t_site, lam, X, Xi):
     return np.minimum((u[:, None] > c).sum(axis=1), P.shape[1] - 1)
 
 
+def _queries_batched(rng, st, tree, site_rate, lam, X, Xi, abet, locs, lens, starts, qpar, S,
+                     ncol, batch=8192):
+    """Substitution-mode queries, a batch at a time: the mid-edge states of
+    every column of a batch's fragments in one _evolve_states call, then
+    i.i.d. substitutions (a different random stream from the per-query loop,
+    so the same seed gives different queries)."""
+    out = []
+    gap = ord("-")
+    nq = len(locs)
+    for b0 in range(0, nq, batch):
+        b1 = min(nq, b0 + batch)
+        v = locs[b0:b1]
+        ln = lens[b0:b1]
+        a = starts[b0:b1]
+        qid = np.repeat(np.arange(b1 - b0), ln)
+        col = np.concatenate([np.arange(x, x + y) for x, y in zip(a, ln)])
+        parent = st[v[qid], col]
+        t = site_rate[col] * (tree.bl[v[qid]] / 2.0)
+        mid = _evolve_states(rng, parent, t, lam, X, Xi)
+        flip = rng.random(len(mid)) < qpar
+        mid[flip] = (mid[flip] + rng.integers(1, S, size=int(flip.sum()))) % S
+        rows = np.full((b1 - b0, ncol), gap, dtype=np.uint8)
+        rows[qid, col] = abet[mid]
+        out.extend(("q%d" % (b0 + k), rows[k].tobytes().decode()) for k in range(b1 - b0))
+    return out
+
+
 def make_workload(name: str, seed: int = 1, n_queries: int | None = None,
-                  n_taxa: int | None = None, n_columns: int | None = None) -> Workload:
+                  n_taxa: int | None = None, n_columns: int | None = None,
+                  batched_queries: bool = False) -> Workload:
+    """The named workload; batched_queries generates substitution-mode
+    queries a batch at a time (for streams of a million queries)."""
     S, taxa, mean_bl, ncol, nq, flo, fhi, qmode, qpar = WORKLOADS[name]
     taxa = n_taxa or taxa
     ncol = n_columns or ncol
@@ -154,6 +184,10 @@ def make_workload(name: str, seed: int = 1, n_queries: int | None = None,
     starts = (rng.random(nq) * (ncol - lens + 1)).astype(np.int64)
     queries = []
     gap = ord("-")
+    if batched_queries and qmode == "subst":
+        queries = _queries_batched(rng, st, tree, site_rate, lam, X, Xi, abet, locs, lens, starts,
+                                   qpar, S, ncol)
+        nq = 0  # skip the per-query loop
     for k in range(nq):
         v = int(locs[k])
         a, b = int(starts[k]), int(starts[k] + lens[k])
