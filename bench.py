@@ -138,6 +138,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--calc-pp", action="store_true",
+                    help="-p: posterior probabilities of the kept placements (keep_at_most 20, "
+                         "keep_factor 0.001, max_strikes 20)")
     ap.add_argument("--stream", action="store_true",
                     help="place the queries as a stream of uneven batches of 1k-50k (one pass per "
                          "form, the tree GLVs resident throughout)")
@@ -169,7 +172,7 @@ def main():
     log("tree ready in %.1f s (G=%d, N=%d); uploading queries" % (time.perf_counter() - t0,
                                                                   wl.tree.n_nodes, N))
     pl.upload_queries(queries)
-    prefs = placer.default_prefs()
+    prefs = placer.default_prefs(calc_pp=args.calc_pp)
     if args.copy_calibration_mb:
         copy_calibration(args.copy_calibration_mb << 20)
         log("copy calibration: %d MiB read + %d MiB written" % (args.copy_calibration_mb,
@@ -189,8 +192,9 @@ def main():
             pl.synchronize()
             times.append(time.perf_counter() - t0)
             stats_l.append(st)
-            log("%s step %d: %.3f s (table %.1f ms, stage1 %.1f ms, pitch %.1f ms, refine %.1f ms)" % (
-                prec_name, i, times[-1], st.t_table_ms, st.t_stage1_ms, st.t_pitch_ms, st.t_refine_ms))
+            log("%s step %d: %.3f s (table %.1f ms, stage1 %.1f ms, pitch %.1f ms, refine %.1f ms%s)" % (
+                prec_name, i, times[-1], st.t_table_ms, st.t_stage1_ms, st.t_pitch_ms, st.t_refine_ms,
+                ", -p %.1f ms" % st.t_pp_ms if args.calc_pp else ""))
         return times, stats_l
 
     forms = {}
@@ -206,7 +210,7 @@ def main():
             "value": st.n_logdots / (ms / 1000.0),
             "queries_per_s": args.queries / (ms / 1000.0),
             "t_table_ms": st.t_table_ms, "t_stage1_ms": st.t_stage1_ms, "t_pitch_ms": st.t_pitch_ms,
-            "t_refine_ms": st.t_refine_ms,
+            "t_refine_ms": st.t_refine_ms, "t_pp_ms": st.t_pp_ms,
             "stage1_ref_equiv_tflops": s1_ref_flops / (st.t_stage1_ms * 1e-3) / 1e12,
             "stage1_gather_GBps": s1_gather_bytes / (st.t_stage1_ms * 1e-3) / 1e9,
             "stage2_tflops": s2_flops / ((st.t_pitch_ms + st.t_refine_ms) * 1e-3) / 1e12,
@@ -266,7 +270,8 @@ def main():
         "data": "synthetic (seeded simulation)",
         "config": {"workload": WORKLOAD_TEXT.get(args.workload, args.workload + ", %d queries") % args.queries,
                    "queries": args.queries, "edges": E, "columns": N,
-                   "form": "plain FP64" if args.precision == "fp64" else "batched FP32"},
+                   "form": "plain FP64" if args.precision == "fp64" else "batched FP32",
+                   "calc_pp": bool(args.calc_pp)},
         "queries_per_s": main_f["queries_per_s"],
         "roofline": roofline,
         "cpu_baseline": cpu,
