@@ -372,42 +372,41 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT>> {
       cat_tables(new_pend, new_dist);
       return;
     }
+    // The exponentials reach the lanes forming P entries through the LDS
+    // staging area sE (one write, a wave fence, independent reads).
     if (new_dist) {
       // exp(rate_r * bl * l_k) for prox (u < 4R) and dist (4R <= u < 8R)
-      double ex = 0.0;
       if (lane < 8 * R) {
         const int which = lane / (4 * R), r = (lane / 4) % R, k = lane % 4;
-        ex = exp((sh->rates[r] * (which ? dist_bl : prox_bl)) * sh->l[k]);
+        sE[lane] = exp((sh->rates[r] * (which ? dist_bl : prox_bl)) * sh->l[k]);
       }
+      wave_lds_sync();
 #pragma unroll
       for (int base = 0; base < 2 * R * 16; base += 64) {
         const int e = base + lane;
         const int which = e / (R * 16), rem = e % (R * 16);
         const int r = rem / 16, i = (rem / 4) % 4, j = rem % 4;
+        const double *ex = sE + (which * R + r) * 4;
         double acc = 0.0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const double exk = __shfl(ex, (which * R + r) * 4 + k, 64);
-          acc = acc + (exk * sh->x[i * 4 + k]) * sh->xit[j * 4 + k];
-        }
+        for (int k = 0; k < 4; k++) acc = acc + (ex[k] * sh->x[i * 4 + k]) * sh->xit[j * 4 + k];
         if (e < 2 * R * 16) (which ? sPd : sPp)[rem] = (Real)acc;
       }
     }
     if (new_pend) {
-      double ex = 0.0;
+      double *sEp = sE + 8 * R;
       if (lane < 4 * R) {
         const int r = lane / 4, k = lane % 4;
-        ex = exp((sh->rates[r] * pend_bl) * sh->l[k]);
+        sEp[lane] = exp((sh->rates[r] * pend_bl) * sh->l[k]);
       }
+      wave_lds_sync();
       // P_pend[r][i][j] at lane r*16 + i*4 + j (R*16 <= 64)
       double pij = 0.0;
       {
         const int r = (lane / 16) % R, i = (lane / 4) % 4, j = lane % 4;
+        const double *ex = sEp + r * 4;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const double exk = __shfl(ex, r * 4 + k, 64);
-          pij = pij + (exk * sh->x[i * 4 + k]) * sh->xit[j * 4 + k];
-        }
+        for (int k = 0; k < 4; k++) pij = pij + (ex[k] * sh->x[i * 4 + k]) * sh->xit[j * 4 + k];
       }
       // One-hot characters: their evolved rows are columns of P (every other
       // product in gemmish_c is an exact zero), row j entry (r, i) = P[r][i][j].
