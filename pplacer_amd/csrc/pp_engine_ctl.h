@@ -22,13 +22,6 @@ namespace pp {
 enum { QNG_OK = 0, QNG_ETOL = 1, QNG_EBADTOL = 2, QNG_ABORT = 3, QNG_ASSERT = 4 };
 enum { PRIOR_EXP = 0, PRIOR_UNIFORM = 1 };
 
-__device__ __forceinline__ double readlane_f64(double v, int l) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
 // Up to 128 wave-uniform doubles spread over the wave: value j lives in lane
 // j % 64 of word j / 64.
 struct LaneStore {
