@@ -68,6 +68,11 @@ __device__ __forceinline__ void prot_copy_model(ProtShared *sh, const DevModel *
   }
 }
 
+// The eigenbasis contractions use fused multiply-adds in both precisions:
+// this grouping of the likelihood already differs from the stubs' rounding.
+__device__ __forceinline__ double emadd(double acc, double a, double b) { return fma(a, b, acc); }
+__device__ __forceinline__ float emadd(float acc, float a, float b) { return fmaf(a, b, acc); }
+
 // ---------------------------------------------------------------------------
 // worker waves
 // ---------------------------------------------------------------------------
@@ -78,7 +83,6 @@ __device__ __forceinline__ void prot_worker(char *smem, Real *scr, int cap, int 
                                             const Real *darr, const Real *parr, const int *darr_e,
                                             const int *parr_e, const int *site_cat) {
   constexpr int S = kProtS, RS = R * S;
-  using Ops = RealOps<Real>;
   ProtShared *sh = reinterpret_cast<ProtShared *>(smem);
   const int *sidx = ProtLayout::sidx(smem);
   const int lane = threadIdx.x & 63;
@@ -112,7 +116,7 @@ __device__ __forceinline__ void prot_worker(char *smem, Real *scr, int cap, int 
           for (int kk = 0; kk < S; kk++) {
             Real acc = (Real)sh->xit[kk] * v[0];
 #pragma unroll
-            for (int j = 1; j < S; j++) acc = Ops::madd(acc, (Real)sh->xit[j * S + kk], v[j]);
+            for (int j = 1; j < S; j++) acc = emadd(acc, (Real)sh->xit[j * S + kk], v[j]);
             Ad[(size_t)kk * cap + p] = acc;
           }
 #pragma unroll
@@ -121,7 +125,7 @@ __device__ __forceinline__ void prot_worker(char *smem, Real *scr, int cap, int 
           for (int kk = 0; kk < S; kk++) {
             Real acc = (Real)sh->xit[kk] * v[0];
 #pragma unroll
-            for (int j = 1; j < S; j++) acc = Ops::madd(acc, (Real)sh->xit[j * S + kk], v[j]);
+            for (int j = 1; j < S; j++) acc = emadd(acc, (Real)sh->xit[j * S + kk], v[j]);
             Ap[(size_t)kk * cap + p] = acc;
           }
           if (r == 0) es += (long long)darr_e[row] + (long long)parr_e[row];
@@ -156,12 +160,12 @@ __device__ __forceinline__ void prot_worker(char *smem, Real *scr, int cap, int 
               Real xi = (Real)xr[0] * bp[0], yi = (Real)xr[0] * bd[0];
 #pragma unroll
               for (int kk = 1; kk < S; kk++) {
-                xi = Ops::madd(xi, (Real)xr[kk], bp[kk]);
-                yi = Ops::madd(yi, (Real)xr[kk], bd[kk]);
+                xi = emadd(xi, (Real)xr[kk], bp[kk]);
+                yi = emadd(yi, (Real)xr[kk], bd[kk]);
               }
               const Real wi = ((Real)sh->statd[i] * xi) * yi;
 #pragma unroll
-              for (int kk = 0; kk < S; kk++) h[kk] = i == 0 ? wi * (Real)xr[kk] : Ops::madd(h[kk], wi, (Real)xr[kk]);
+              for (int kk = 0; kk < S; kk++) h[kk] = i == 0 ? wi * (Real)xr[kk] : emadd(h[kk], wi, (Real)xr[kk]);
             }
             const int c = qc[sidx[m]];
             util = 0;
@@ -169,17 +173,17 @@ __device__ __forceinline__ void prot_worker(char *smem, Real *scr, int cap, int 
             for (int kk = 0; kk < S; kk++) {
               const Real g = (Real)sh->lt[c * S + kk] * h[kk];
               if (act) Gs[(size_t)kk * cap + pc] = g;
-              util = kk == 0 ? (Real)sh->en[er * S] * g : Ops::madd(util, (Real)sh->en[er * S + kk], g);
+              util = kk == 0 ? (Real)sh->en[er * S] * g : emadd(util, (Real)sh->en[er * S + kk], g);
             }
           } else {
             util = (Real)sh->en[er * S] * Gs[pc];
 #pragma unroll
-            for (int kk = 1; kk < S; kk++) util = Ops::madd(util, (Real)sh->en[er * S + kk], Gs[(size_t)kk * cap + pc]);
+            for (int kk = 1; kk < S; kk++) util = emadd(util, (Real)sh->en[er * S + kk], Gs[(size_t)kk * cap + pc]);
           }
           double u = act ? (double)util : 0.0;
-          if (R == 4) {
-            u += __shfl_xor(u, 1, 64);
-            u += __shfl_xor(u, 2, 64);
+          if (R == 4) {  // the site's four rates sit in one quad of lanes
+            u += dpp_f64<0xB1>(u);  // lane ^ 1
+            u += dpp_f64<0x4E>(u);  // lane ^ 2
           }
           if (act && r == 0) lsum += log(u);
         }
