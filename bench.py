@@ -112,16 +112,23 @@ def cpu_baseline(wl, queries, n_sample, threads):
     pre.pull_and_supernodes()
     core = po.Core(pre)
     sample = queries[:n_sample]
+    E = wl.tree.n_nodes - 1
     t0 = time.perf_counter()
     st, res, status, stats = core.place_many(sample, n_threads=threads)
     dt = time.perf_counter() - t0
-    E = wl.tree.n_nodes - 1
+    # the same oracle on one thread (pplacer -j 1), over the first eighth of the sample
+    one = sample[:max(1, len(sample) // 8)]
+    t1 = time.perf_counter()
+    core.place_many(one, n_threads=1)
+    dt1 = time.perf_counter() - t1
     return {"value": len(sample) * E / dt, "unit": "query x edge evals/s", "cores": threads,
             "kind": "port",
             "sample": "%d queries of the same workload (FP64 oracle, %d threads, %.1f s wall, "
                       "%.1f ms/query/thread)" % (len(sample), threads, dt,
                                                   1000 * dt * threads / len(sample)),
-            "queries_per_s": len(sample) / dt}
+            "queries_per_s": len(sample) / dt,
+            "one_thread": {"value": len(one) * E / dt1, "queries_per_s": len(one) / dt1,
+                           "sample": "%d queries, 1 thread, %.1f s" % (len(one), dt1)}}
 
 
 def main():
