@@ -46,7 +46,11 @@ namespace {
 constexpr int kTQ = 16;           // queries per stage-1 tile
 constexpr int kChunk = 32768;     // queries per device batch
 constexpr int kBuckets = 30;      // stage-2 masked-length classes of 64 sites
-constexpr int kBucketStreams = 4; // streams running a stage-2 phase's buckets concurrently
+constexpr int kBucketStreams = 8; // streams for running a stage-2 phase's buckets concurrently
+// Streams a 4-state stage-2 phase spreads its buckets over (0: the main stream
+// only), by form [FP64, FP32].  Measured at 100,000 queries, 2,000 taxa.
+constexpr int kPitchStreams[2] = {0, 8};
+constexpr int kRefineStreams[2] = {4, 8};
 constexpr int kProtBlock = 512;   // lanes of a 20-state stage-2 workgroup (master + workers)
 
 struct DBuf {
@@ -843,9 +847,9 @@ struct LaunchStream {
 // A placement pass over the resident queries, in chunks.  Stage 1 of every
 // chunk runs on its own stream into one of two score buffers, so it overlaps
 // stage 2 of the previous chunk; stage 2 (baseball, keep, refinement, -p,
-// finalize) runs chunk after chunk on the main stream; in the 4-state FP32
-// form the buckets of each phase run concurrently on kBucketStreams streams
-// (the tails of their work queues overlap), joined before the next phase.
+// finalize) runs chunk after chunk on the main stream; in the 4-state form a
+// phase's buckets can run concurrently on up to kBucketStreams streams (the
+// tails of their work queues overlap), joined before the next phase.
 int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats) {
   CHECK(prefs, PP_EINVAL, "prefs is null");
   CHECK(precision == PP_FP64 || precision == PP_FP32, PP_EINVAL, "precision must be PP_FP64 or PP_FP32");
@@ -859,7 +863,13 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   else { t_max_pitches = prefs->max_pitches; t_max_strikes = prefs->max_strikes; }
   const int cap = (int)std::min<long long>(E, std::max<long long>(1, t_max_pitches + 1));
   const int chunk = std::max(1, std::min(Q, (int)std::min<long long>(kChunk, (1LL << 31) / ((long long)E * 8 + 1))));
+  // Chunk boundaries cq0[0 .. n_chunks]: the fewest chunks of at most `chunk`
+  // queries, of even size.  Against full chunks and a short last one, at
+  // 100,000 queries: FP32 250 -> 245 ms per pass, FP64 356 -> 359 ms; a short
+  // first chunk (its stage 1 overlaps nothing) measured no faster.
   const int n_chunks = Q > 0 ? (Q + chunk - 1) / chunk : 0;
+  std::vector<int> cq0(n_chunks + 1, 0);
+  for (int k = 1; k <= n_chunks; k++) cq0[k] = (int)((long long)Q * k / n_chunks);
   pp_placement *d_out;
   int *d_nout, *d_status;
   ENSURE(c->b_out, (size_t)std::max(Q, 1) * K, &d_out);
@@ -891,11 +901,13 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   LaunchStream ls(c, c->stream);
   const bool fp32 = precision == PP_FP32;
   const bool s2_fp32 = fp32 && stage2_fp32_ok(c);
-  // Concurrent buckets in the FP32 form (100,000 queries: 323 -> 274 ms per
-  // pass).  In the FP64 form they slow each other's spilling FP64 kernels
-  // (383 -> 386 ms, against 374 with stage-1 overlap alone), and the 20-state
-  // kernels share one scratch buffer.
-  const bool fork = c->model.S == 4 && s2_fp32;
+  // Concurrent buckets, measured at 100,000 queries: the FP32 form 323 -> 274
+  // ms per pass with 4 streams, 258 -> 251 -> 245 ms with 8 for its baseball
+  // and then for its refinement; the FP64 refinement 364 -> 356 ms with 4 (8:
+  // no further gain).  The FP64 baseball kernels slow each other (2 streams:
+  // 364 -> 372 ms), and the 20-state kernels share one scratch buffer.
+  const int pitch_streams = c->model.S == 4 ? kPitchStreams[s2_fp32 ? 1 : 0] : 0;
+  const int refine_streams = c->model.S == 4 ? kRefineStreams[s2_fp32 ? 1 : 0] : 0;
   hipEvent_t e_t0 = pool.get(), e_t1 = pool.get();
   CHECK(e_t0 && e_t1, PP_EDEVICE, "hipEventCreate failed");
   HIPCHECK(hipEventRecord(e_t0, c->stream));
@@ -912,7 +924,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   std::vector<int> t_lo, t_hi, t_q, t_off(n_chunks + 1, 0), qlist_all(std::max(Q, 1), -1);
   std::vector<std::vector<int>> boffs(n_chunks, std::vector<int>(kBuckets + 1, 0));
   for (int ci = 0; ci < n_chunks; ci++) {
-    const int q0 = ci * chunk, nq = std::min(chunk, Q - q0);
+    const int q0 = cq0[ci], nq = cq0[ci + 1] - q0;
     t_off[ci] = (int)t_lo.size();
     stage1_tiles(c, q0, nq, t_lo, t_hi, t_q);
     std::vector<std::vector<int>> buckets(kBuckets);
@@ -948,10 +960,11 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   struct Marks { hipEvent_t s1a, s1b, p0, p1, r1, m1, f1; };
   std::vector<Marks> marks(n_chunks);
   std::vector<hipEvent_t> scores_free(n_sbuf, nullptr);  // last pitch phase reading each buffer
-  // Runs launch(b) for every non-empty bucket, concurrently on the bucket
-  // streams when forking, and joins them back into the main stream.
-  auto phase = [&](const std::vector<int> &boff, auto &&launch) -> int {
-    if (!fork) {
+  // Runs launch(b) for every non-empty bucket, round-robin over the first
+  // n_streams bucket streams (none: the main stream), and joins them back
+  // into the main stream.
+  auto phase = [&](const std::vector<int> &boff, int n_streams, auto &&launch) -> int {
+    if (n_streams <= 0) {
       ls.set(c->stream);
       for (int b = 0; b < kBuckets; b++)
         if (boff[b + 1] > boff[b]) {
@@ -967,7 +980,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     int k = 0;
     for (int b = 0; b < kBuckets; b++) {
       if (boff[b + 1] == boff[b]) continue;
-      const int si = k++ % kBucketStreams;
+      const int si = k++ % std::min(n_streams, kBucketStreams);
       if (!used[si]) {
         HIPCHECK(hipStreamWaitEvent(c->bs[si], ef, 0));
         used[si] = true;
@@ -987,7 +1000,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     return PP_OK;
   };
   for (int ci = 0; ci < n_chunks; ci++) {
-    const int q0 = ci * chunk, nq = std::min(chunk, Q - q0);
+    const int q0 = cq0[ci], nq = cq0[ci + 1] - q0;
     double *scores = d_scores + (size_t)(ci % n_sbuf) * chunk * E;
     Marks &mk = marks[ci];
     for (hipEvent_t *e : {&mk.s1a, &mk.s1b, &mk.p0, &mk.p1, &mk.r1, &mk.m1, &mk.f1}) {
@@ -1013,7 +1026,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     HIPCHECK(hipEventRecord(mk.p0, c->stream));
     const uint8_t *codes = reinterpret_cast<const uint8_t *>(c->b_codes.p) + (size_t)q0 * N;
     int *status = d_status + q0;
-    st = phase(boff, [&](int b) -> int {
+    st = phase(boff, pitch_streams, [&](int b) -> int {
       PitchArgs p{};
       p.model = c->d_model;
       p.codes = codes;
@@ -1059,7 +1072,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     ka.n_keep = d_nkeep;
     hipLaunchKernelGGL(keep_kernel, dim3((nq + 255) / 256), dim3(256), 0, c->stream, ka);
     HIPCHECK(hipGetLastError());
-    st = phase(boff, [&](int b) -> int {
+    st = phase(boff, refine_streams, [&](int b) -> int {
       OptArgs o{};
       std::memset(&o, 0, sizeof(o));
       o.model = c->d_model;
@@ -1088,7 +1101,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     if (st) return st;
     HIPCHECK(hipEventRecord(mk.r1, c->stream));
     if (prefs->calc_pp) {  // core.ml:378-392, one job per kept placement
-      st = phase(boff, [&](int b) -> int {
+      st = phase(boff, refine_streams, [&](int b) -> int {
         OptArgs o = marg_proto;
         o.model = c->d_model;
         o.codes = codes;
