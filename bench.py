@@ -151,6 +151,9 @@ def main():
     ap.add_argument("--stream", action="store_true",
                     help="place the queries as a stream of uneven batches of 1k-50k (one pass per "
                          "form, the tree GLVs resident throughout)")
+    ap.add_argument("--tune", action="store_true",
+                    help="tune each form's launch shape (bucket streams per stage-2 phase, chunk "
+                         "size) on the resident queries before its warmup (untimed)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size "
                          "kernel for scaling hardware byte counters)")
@@ -187,6 +190,10 @@ def main():
 
     def run_form(prec_name):
         prec = PP_FP64 if prec_name == "fp64" else PP_FP32
+        if args.tune:
+            from pplacer_amd import tune
+            best, trials = tune.tune(pl, prefs, prec, log=log)
+            log("%s launch shape tuned over %d trials: %s" % (prec_name, len(trials), best))
         for i in range(args.warmup):
             log("%s warmup %d" % (prec_name, i))
             pl.place_resident(prefs, prec)
@@ -226,6 +233,7 @@ def main():
             "pitch_avg_launch_ms": st.t_pitch_ms / max(1, st.n_pitch_launches),
             "n_logdots": st.n_logdots, "n_like_calls": st.n_like_calls,
             "n_pitched": st.n_pitched, "n_launches": st.n_launches,
+            "launch": pl.launch_config(), "launch_tuned": bool(args.tune),
         }
     main_f = forms[args.precision]
     st_main = main_f

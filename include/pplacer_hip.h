@@ -268,6 +268,28 @@ int pp_place_resident(pp_ctx *ctx, const pp_prefs *prefs, int precision, pp_stat
 int pp_results_download(pp_ctx *ctx, pp_placement *out, int *n_out, int *status);
 int pp_synchronize(pp_ctx *ctx);
 
+/* Launch shape of a placement pass; no counterpart in the reference (its
+ * per-query loop has no launches).  For tuners: results do not depend on it.
+ *   pitch_streams[f], refine_streams[f]: streams over which the 4-state
+ *     baseball phase and the refinement (and -p) phase spread their masked-
+ *     length buckets, for form f (0 = PP_FP64, 1 = PP_FP32); 0 runs them one
+ *     after another on the main stream; at most PP_MAX_BUCKET_STREAMS.
+ *     20-state passes ignore them and run on the main stream.
+ *   chunk: queries per device chunk (stage 1 of the next chunk overlaps
+ *     stage 2 of the current one), 1 .. PP_MAX_CHUNK, further capped so that a
+ *     chunk's scores ([chunk][n_edges] doubles) stay under 2 GiB.
+ * pp_launch_config_set returns PP_EINVAL, changing nothing, for a value out
+ * of range. */
+#define PP_MAX_BUCKET_STREAMS 8
+#define PP_MAX_CHUNK 262144
+typedef struct {
+  int pitch_streams[2];
+  int refine_streams[2];
+  int chunk;
+} pp_launch_config;
+int pp_launch_config_get(pp_ctx *ctx, pp_launch_config *out);
+int pp_launch_config_set(pp_ctx *ctx, const pp_launch_config *cfg);
+
 #ifdef __cplusplus
 }
 #endif

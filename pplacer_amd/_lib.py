@@ -50,6 +50,13 @@ class Stats(C.Structure):
                 ("n_pitch_launches", C.c_int64), ("t_pp_ms", C.c_double)]
 
 
+class LaunchConfig(C.Structure):
+    _fields_ = [("pitch_streams", C.c_int * 2), ("refine_streams", C.c_int * 2),
+                ("chunk", C.c_int)]
+
+
+PP_MAX_BUCKET_STREAMS, PP_MAX_CHUNK = 8, 262144
+
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _i64p = C.POINTER(C.c_int64)
@@ -94,6 +101,8 @@ SIGNATURES = {
     "pp_place_resident": (_i, [_vp, C.POINTER(Prefs), _i, C.POINTER(Stats)]),
     "pp_results_download": (_i, [_vp, C.POINTER(Placement), _ip, _ip]),
     "pp_synchronize": (_i, [_vp]),
+    "pp_launch_config_get": (_i, [_vp, C.POINTER(LaunchConfig)]),
+    "pp_launch_config_set": (_i, [_vp, C.POINTER(LaunchConfig)]),
 }
 
 _lib = None

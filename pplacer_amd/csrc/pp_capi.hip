@@ -46,10 +46,12 @@ namespace {
 constexpr int kTQ = 16;           // queries per stage-1 tile
 constexpr int kChunk = 32768;     // queries per device batch
 constexpr int kBuckets = 30;      // stage-2 masked-length classes of 64 sites
-constexpr int kBucketStreams = 8; // streams for running a stage-2 phase's buckets concurrently
-// Streams a 4-state stage-2 phase spreads its buckets over (0: the main stream
-// only), by form [FP64, FP32].  Measured at 100,000 queries, 2,000 taxa.
-constexpr int kPitchStreams[2] = {0, 8};
+constexpr int kBucketStreams = PP_MAX_BUCKET_STREAMS;  // streams for a stage-2 phase's buckets
+static_assert(kChunk <= PP_MAX_CHUNK, "default chunk out of range");
+// Default streams a 4-state stage-2 phase spreads its buckets over (0: the
+// main stream only), by form [FP64, FP32].  Measured at 100,000 queries,
+// 2,000 taxa.
+constexpr int kPitchStreams[2] = {8, 8};
 constexpr int kRefineStreams[2] = {4, 8};
 constexpr int kProtBlock = 512;   // lanes of a 20-state stage-2 workgroup (master + workers)
 
@@ -92,6 +94,7 @@ struct pp_ctx {
   hipStream_t s1 = nullptr;           // stage 1 of the next chunk
   hipStream_t bs[kBucketStreams] = {};  // concurrent stage-2 buckets
   hipStream_t cur = nullptr;          // the stream the launch helpers use
+  pp_launch_config launch{{kPitchStreams[0], kPitchStreams[1]}, {kRefineStreams[0], kRefineStreams[1]}, kChunk};
   HostModel model;
   DBuf b_model;
   DevModel *d_model = nullptr;
@@ -862,7 +865,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   if (prefs->max_strikes == 0) { t_max_pitches = 1LL << 62; t_max_strikes = 1LL << 62; }
   else { t_max_pitches = prefs->max_pitches; t_max_strikes = prefs->max_strikes; }
   const int cap = (int)std::min<long long>(E, std::max<long long>(1, t_max_pitches + 1));
-  const int chunk = std::max(1, std::min(Q, (int)std::min<long long>(kChunk, (1LL << 31) / ((long long)E * 8 + 1))));
+  const int chunk = std::max(1, std::min(Q, (int)std::min<long long>(c->launch.chunk, (1LL << 31) / ((long long)E * 8 + 1))));
   // Chunk boundaries cq0[0 .. n_chunks]: the fewest chunks of at most `chunk`
   // queries, of even size.  Against full chunks and a short last one, at
   // 100,000 queries: FP32 250 -> 245 ms per pass, FP64 356 -> 359 ms; a short
@@ -904,10 +907,10 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   // Concurrent buckets, measured at 100,000 queries: the FP32 form 323 -> 274
   // ms per pass with 4 streams, 258 -> 251 -> 245 ms with 8 for its baseball
   // and then for its refinement; the FP64 refinement 364 -> 356 ms with 4 (8:
-  // no further gain).  The FP64 baseball kernels slow each other (2 streams:
-  // 364 -> 372 ms), and the 20-state kernels share one scratch buffer.
-  const int pitch_streams = c->model.S == 4 ? kPitchStreams[s2_fp32 ? 1 : 0] : 0;
-  const int refine_streams = c->model.S == 4 ? kRefineStreams[s2_fp32 ? 1 : 0] : 0;
+  // no further gain), the FP64 baseball 360 -> 346 ms with 8 (2 or 4: no
+  // gain).  The 20-state kernels share one scratch buffer and do not fork.
+  const int pitch_streams = c->model.S == 4 ? c->launch.pitch_streams[s2_fp32 ? 1 : 0] : 0;
+  const int refine_streams = c->model.S == 4 ? c->launch.refine_streams[s2_fp32 ? 1 : 0] : 0;
   hipEvent_t e_t0 = pool.get(), e_t1 = pool.get();
   CHECK(e_t0 && e_t1, PP_EDEVICE, "hipEventCreate failed");
   HIPCHECK(hipEventRecord(e_t0, c->stream));
@@ -1446,6 +1449,25 @@ int pp_results_download(pp_ctx *c, pp_placement *out, int *n_out, int *status) {
 int pp_synchronize(pp_ctx *c) {
   CHECK(c, PP_EINVAL, "null context");
   HIPCHECK(hipStreamSynchronize(c->stream));
+  return PP_OK;
+}
+
+int pp_launch_config_get(pp_ctx *c, pp_launch_config *out) {
+  CHECK(c && out, PP_EINVAL, "null argument");
+  *out = c->launch;
+  return PP_OK;
+}
+
+int pp_launch_config_set(pp_ctx *c, const pp_launch_config *cfg) {
+  CHECK(c && cfg, PP_EINVAL, "null argument");
+  for (int f = 0; f < 2; f++) {
+    CHECK(cfg->pitch_streams[f] >= 0 && cfg->pitch_streams[f] <= kBucketStreams, PP_EINVAL,
+          "pitch_streams must be in 0..PP_MAX_BUCKET_STREAMS");
+    CHECK(cfg->refine_streams[f] >= 0 && cfg->refine_streams[f] <= kBucketStreams, PP_EINVAL,
+          "refine_streams must be in 0..PP_MAX_BUCKET_STREAMS");
+  }
+  CHECK(cfg->chunk >= 1 && cfg->chunk <= PP_MAX_CHUNK, PP_EINVAL, "chunk must be in 1..PP_MAX_CHUNK");
+  c->launch = *cfg;
   return PP_OK;
 }
 

@@ -15,7 +15,7 @@ import numpy as np
 
 from . import refio
 from ._lib import (PP_FP32, PP_FP64, PP_PRIOR_FLAT_EXP, PP_PRIOR_INFORMATIVE, PP_PRIOR_UNIFORM,
-                   Placement, Prefs, Stats, check, lib)
+                   LaunchConfig, Placement, Prefs, Stats, check, lib)
 
 _dp = C.POINTER(C.c_double)
 
@@ -294,6 +294,26 @@ class Placer:
 
     def synchronize(self):
         check(lib().pp_synchronize(self._ctx), "synchronize")
+
+    def launch_config(self) -> dict:
+        """The pass's launch shape: streams per stage-2 phase and form
+        (index 0 = FP64, 1 = FP32) and queries per device chunk."""
+        cfg = LaunchConfig()
+        check(lib().pp_launch_config_get(self._ctx, C.byref(cfg)), "launch_config_get")
+        return {"pitch_streams": list(cfg.pitch_streams), "refine_streams": list(cfg.refine_streams),
+                "chunk": cfg.chunk}
+
+    def set_launch_config(self, pitch_streams=None, refine_streams=None, chunk=None) -> dict:
+        """Changes the given fields of the launch shape (results do not
+        depend on it); returns the new shape."""
+        cur = self.launch_config()
+        cfg = LaunchConfig()
+        for k, v in (("pitch_streams", pitch_streams), ("refine_streams", refine_streams)):
+            v = cur[k] if v is None else list(v)
+            getattr(cfg, k)[0], getattr(cfg, k)[1] = int(v[0]), int(v[1])
+        cfg.chunk = int(cur["chunk"] if chunk is None else chunk)
+        check(lib().pp_launch_config_set(self._ctx, C.byref(cfg)), "launch_config_set")
+        return self.launch_config()
 
 
 def setup_from_workload(wl, device=0, premasked=True):
