@@ -229,8 +229,10 @@ def main():
             "stage1_gather_GBps": s1_gather_bytes / (st.t_stage1_ms * 1e-3) / 1e9,
             "stage2_tflops": s2_flops / ((st.t_pitch_ms + st.t_refine_ms) * 1e-3) / 1e12,
             "pitch_u2_tflops": pitch_flops(st, S=S) / (st.t_pitch_ms * 1e-3) / 1e12,
+            "pitch_u2_tflops_per_launch": pitch_flops(st, S=S) / (st.t_pitch_kernel_ms * 1e-3) / 1e12,
             "pitch_launches": st.n_pitch_launches,
-            "pitch_avg_launch_ms": st.t_pitch_ms / max(1, st.n_pitch_launches),
+            "pitch_avg_launch_ms": st.t_pitch_kernel_ms / max(1, st.n_pitch_launches),
+            "pitch_phase_ms_per_launch": st.t_pitch_ms / max(1, st.n_pitch_launches),
             "n_logdots": st.n_logdots, "n_like_calls": st.n_like_calls,
             "n_pitched": st.n_pitched, "n_launches": st.n_launches,
             "launch": pl.launch_config(), "launch_tuned": bool(args.tune),
@@ -257,7 +259,11 @@ def main():
         name = ("pitch_mw_kernel<double,4,*,1,false>" if S == 4 and real == "double"
                 else "pitch_mw_kernel<float,4,512,2,false>" if S == 4
                 else "pitch_prot_kernel<double,4>")
-        achieved = st_main["pitch_u2_tflops"]
+        # per launch: the flops of the phase's launches over the sum of their
+        # own durations (HIP events around each launch, on its stream); the
+        # launches of one phase run concurrently on several streams, so the
+        # phase as a whole reaches phase_achieved
+        achieved = st_main["pitch_u2_tflops_per_launch"]
         roofline = {"bound": "mfma",
                     "pipe": ("vector ALU %s (4-state contractions do not use MFMA)" % args.precision
                              if S == 4 else "vector ALU fp64 (eigenbasis evolves)"),
@@ -265,6 +271,8 @@ def main():
                     "frac": achieved / peak, "traffic": read_traffic(name),
                     "avg_launch_ms": st_main["pitch_avg_launch_ms"],
                     "launches": st_main["pitch_launches"],
+                    "phase_achieved": st_main["pitch_u2_tflops"],
+                    "phase_frac": st_main["pitch_u2_tflops"] / peak,
                     "work": "reference U2 flops of the pitched probes: (2RS^2+4RS) per masked site "
                             "of a pendant probe, (4RS^2+4RS) of a distal probe"}
         if kern != "pitch_kernel":

@@ -205,6 +205,9 @@ typedef struct {
   int64_t n_site_probes_dist_pitch; /* n_site_probes_dist of the baseball kernels alone */
   int64_t n_pitch_launches;         /* launches of the baseball kernel */
   double t_pp_ms;                   /* -p marginal likelihoods ("PP calculation", core.ml:380) */
+  double t_pitch_kernel_ms;         /* sum over baseball launches of each launch's own
+                                       duration (launches on concurrent streams overlap,
+                                       so this can exceed t_pitch_ms) */
 } pp_stats;
 
 /* Stage 1 for a batch: scores[q][e] = Glv.masked_logdot of the query (full

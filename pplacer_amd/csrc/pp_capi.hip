@@ -51,7 +51,7 @@ static_assert(kChunk <= PP_MAX_CHUNK, "default chunk out of range");
 // Default streams a 4-state stage-2 phase spreads its buckets over (0: the
 // main stream only), by form [FP64, FP32].  Measured at 100,000 queries,
 // 2,000 taxa.
-constexpr int kPitchStreams[2] = {8, 8};
+constexpr int kPitchStreams[2] = {0, 8};
 constexpr int kRefineStreams[2] = {4, 8};
 constexpr int kProtBlock = 512;   // lanes of a 20-state stage-2 workgroup (master + workers)
 
@@ -907,8 +907,10 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   // Concurrent buckets, measured at 100,000 queries: the FP32 form 323 -> 274
   // ms per pass with 4 streams, 258 -> 251 -> 245 ms with 8 for its baseball
   // and then for its refinement; the FP64 refinement 364 -> 356 ms with 4 (8:
-  // no further gain), the FP64 baseball 360 -> 346 ms with 8 (2 or 4: no
-  // gain).  The 20-state kernels share one scratch buffer and do not fork.
+  // no further gain).  The FP64 baseball kernels stay on the main stream by
+  // default, so that each launch's duration is the kernel's own; 8 streams
+  // measured 360 -> 346 ms on one host (2 or 4: no gain), and a tuner may set
+  // them.  The 20-state kernels share one scratch buffer and do not fork.
   const int pitch_streams = c->model.S == 4 ? c->launch.pitch_streams[s2_fp32 ? 1 : 0] : 0;
   const int refine_streams = c->model.S == 4 ? c->launch.refine_streams[s2_fp32 ? 1 : 0] : 0;
   hipEvent_t e_t0 = pool.get(), e_t1 = pool.get();
@@ -963,6 +965,8 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   struct Marks { hipEvent_t s1a, s1b, p0, p1, r1, m1, f1; };
   std::vector<Marks> marks(n_chunks);
   std::vector<hipEvent_t> scores_free(n_sbuf, nullptr);  // last pitch phase reading each buffer
+  // each baseball launch's own interval, on the stream it was launched on
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pitch_ev;
   // Runs launch(b) for every non-empty bucket, round-robin over the first
   // n_streams bucket streams (none: the main stream), and joins them back
   // into the main stream.
@@ -1058,7 +1062,13 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
       p.n_pitched = d_stats + 1;
       launches++;
       pitch_launches++;
-      return s2_fp32 ? dispatch_pitch<float>(c, p, b) : dispatch_pitch<double>(c, p, b);
+      hipEvent_t k0 = pool.get(), k1 = pool.get();
+      CHECK(k0 && k1, PP_EDEVICE, "hipEventCreate failed");
+      HIPCHECK(hipEventRecord(k0, c->cur));
+      const int r = s2_fp32 ? dispatch_pitch<float>(c, p, b) : dispatch_pitch<double>(c, p, b);
+      HIPCHECK(hipEventRecord(k1, c->cur));
+      pitch_ev.emplace_back(k0, k1);
+      return r;
     });
     if (st) return st;
     HIPCHECK(hipEventRecord(mk.p1, c->stream));
@@ -1162,6 +1172,12 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     t_ref += a3 + a5;
     t_pp += a4;
   }
+  double t_pitch_k = 0;
+  for (const auto &kv : pitch_ev) {
+    float a = 0;
+    HIPCHECK(hipEventElapsedTime(&a, kv.first, kv.second));
+    t_pitch_k += a;
+  }
   if (stats) {
     unsigned long long hs[6];
     HIPCHECK(hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost));
@@ -1186,6 +1202,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     stats->n_site_probes_dist_pitch = (int64_t)hs[5];
     stats->n_pitch_launches = pitch_launches;
     stats->t_pp_ms = t_pp;
+    stats->t_pitch_kernel_ms = t_pitch_k;
     stats->n_launches = launches;
   }
   return PP_OK;

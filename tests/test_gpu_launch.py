@@ -15,7 +15,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 NQ = 3000
-DEFAULT = {"pitch_streams": [8, 8], "refine_streams": [4, 8], "chunk": 32768}
+DEFAULT = {"pitch_streams": [0, 8], "refine_streams": [4, 8], "chunk": 32768}
 
 
 def _print_case(name):

@@ -12,7 +12,7 @@ class TimedShapes:
     """place_resident advances a virtual clock by cost(shape, form)."""
 
     def __init__(self, cost, monkeypatch):
-        self.cfg = {"pitch_streams": [8, 8], "refine_streams": [4, 8], "chunk": 32768}
+        self.cfg = {"pitch_streams": [0, 8], "refine_streams": [4, 8], "chunk": 32768}
         self.cost = cost
         self.now = 0.0
         self.passes = 0
@@ -45,12 +45,12 @@ def test_tune_finds_the_fastest_shape(monkeypatch):
                 + (0 if cfg["chunk"] == 65536 else 5))
     pl = TimedShapes(cost, monkeypatch)
     best, trials = tune.tune(pl, None, PP_FP32, reps=1)
-    assert best == {"pitch_streams": [8, 4], "refine_streams": [4, 2], "chunk": 65536}
+    assert best == {"pitch_streams": [0, 4], "refine_streams": [4, 2], "chunk": 65536}
     assert pl.launch_config() == best
     # the other form's entries are untouched
-    assert best["pitch_streams"][0] == 8 and best["refine_streams"][0] == 4
+    assert best["pitch_streams"][0] == 0 and best["refine_streams"][0] == 4
     # the first trial is the starting shape; every trial timed one pass
-    assert trials[0][0] == {"pitch_streams": [8, 8], "refine_streams": [4, 8], "chunk": 32768}
+    assert trials[0][0] == {"pitch_streams": [0, 8], "refine_streams": [4, 8], "chunk": 32768}
     assert pl.passes == 1 + len(trials)
 
 
@@ -60,7 +60,7 @@ def test_tune_ignores_gains_within_the_margin(monkeypatch):
         return 200 - (1.0 if cfg["pitch_streams"][f] == 2 else 0.0)
     pl = TimedShapes(cost, monkeypatch)
     best, _ = tune.tune(pl, None, PP_FP64, reps=1)
-    assert best == {"pitch_streams": [8, 8], "refine_streams": [4, 8], "chunk": 32768}
+    assert best == {"pitch_streams": [0, 8], "refine_streams": [4, 8], "chunk": 32768}
 
 
 @pytest.mark.parametrize("reps", [1, 3])
