@@ -820,8 +820,10 @@ int queries_upload(pp_ctx *c, int n_queries, const char *queries) {
   int *d_qM;
   ENSURE(c->b_codes, (size_t)n_queries * N, &d_codes);
   ENSURE(c->b_qM, n_queries, &d_qM);
-  HIPCHECK(hipMemcpy(d_codes, sorted.data(), sorted.size(), hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(d_qM, c->qM.data(), (size_t)n_queries * 4, hipMemcpyHostToDevice));
+  if (n_queries > 0) {
+    HIPCHECK(hipMemcpy(d_codes, sorted.data(), sorted.size(), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(d_qM, c->qM.data(), (size_t)n_queries * 4, hipMemcpyHostToDevice));
+  }
   return PP_OK;
 }
 
@@ -891,7 +893,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   ENSURE(c->b_counter, 128, &d_counter);
   ENSURE(c->b_stats, 6, &d_stats);
   HIPCHECK(hipMemsetAsync(d_stats, 0, 6 * sizeof(unsigned long long), c->stream));
-  HIPCHECK(hipMemcpyAsync(d_status, c->qstat.data(), (size_t)Q * 4, hipMemcpyHostToDevice, c->stream));
+  if (Q > 0) HIPCHECK(hipMemcpyAsync(d_status, c->qstat.data(), (size_t)Q * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemsetAsync(d_nout, 0, (size_t)std::max(Q, 1) * 4, c->stream));
   long long launches = 0, pitch_launches = 0;
   OptArgs marg_proto{};
