@@ -274,10 +274,10 @@ int pp_synchronize(pp_ctx *ctx);
 /* Launch shape of a placement pass; no counterpart in the reference (its
  * per-query loop has no launches).  For tuners: results do not depend on it.
  *   pitch_streams[f], refine_streams[f]: streams over which the 4-state
- *     baseball phase and the refinement (and -p) phase spread their masked-
- *     length buckets, for form f (0 = PP_FP64, 1 = PP_FP32); 0 runs them one
- *     after another on the main stream; at most PP_MAX_BUCKET_STREAMS.
- *     20-state passes ignore them and run on the main stream.
+ *     baseball phase and the refinement phase spread their masked-length
+ *     buckets, for form f (0 = PP_FP64, 1 = PP_FP32); 0 runs them one after
+ *     another on the main stream; at most PP_MAX_BUCKET_STREAMS.  The -p
+ *     phase and 20-state passes always run on the main stream.
  *   chunk: queries per device chunk (stage 1 of the next chunk overlaps
  *     stage 2 of the current one), 1 .. PP_MAX_CHUNK, further capped so that a
  *     chunk's scores ([chunk][n_edges] doubles) stay under 2 GiB.
