@@ -185,24 +185,51 @@ struct ThreeTaxControl {
     if (left >= raw_start || raw_start >= right) return ST_INVALID_START;
     const double righty = fprobe(kind, right, err);  // (f left, f right) evaluates right first
     if (err) return err;
+    const double prox_r = prox_bl;
     const double lefty = fprobe(kind, left, err);
     if (err) return err;
+    const double prox_l = prox_bl;
     const double smaller = lefty < righty ? left : right;
     const double miny = lefty <= righty ? lefty : righty;  // Stdlib.min
-    double prev = raw_start, pv;
+    double prev = raw_start, pv, prox_p;
     for (int iter = 1;; iter++) {
       pv = fprobe(kind, prev, err);
       if (err) return err;
+      prox_p = prox_bl;
       if (pv < miny) break;
       if (fabs(prev - smaller) < tol) { *result = smaller; return ST_OK; }  // FoundMin
       if (iter > max_iters) return ST_FIND_START;
       prev = (prev + smaller) / 2.;
     }
-    // gsl_min_fminimizer_set: f(lower), f(upper), f(start) are the values just
-    // computed (the three-taxon state is already at start, as after the set).
+    // gsl_min_fminimizer_set evaluates f(lower), f(upper), f(start) again.  A
+    // pendant probe moves only the pendant length, so those are the values
+    // just computed, and the state is already at start.  A distal probe also
+    // moves the proximal length to (dist + prox) - v, whose rounding drifts
+    // from probe to probe: the three moves are replayed in the set's order,
+    // with their assertion, and a value is reused only where the proximal
+    // length comes out bit-identical to the one it was computed at.
     double x_lower = left, x_upper = right, x_min = prev;
     double f_lower = lefty, f_upper = righty, f_min = pv;
-    if (!isfinite(f_lower) || !isfinite(f_upper) || !isfinite(f_min)) return ST_GSL_BADFUNC;
+    // (scalar selects and one probe site: no addressable locals in the loop)
+#pragma nounroll
+    for (int k = 0; k < 3; k++) {
+      const double x = k == 0 ? left : k == 1 ? right : prev;
+      double f = k == 0 ? f_lower : k == 1 ? f_upper : f_min;
+      if (kind == 1) {
+        const double prox = (dist_bl + prox_bl) - x;
+        if (prox == (k == 0 ? prox_l : k == 1 ? prox_r : prox_p)) {
+          dist_bl = x;
+          prox_bl = prox;
+        } else {
+          f = fprobe(kind, x, err);
+          if (err) return err;
+        }
+      }
+      if (!isfinite(f)) return ST_GSL_BADFUNC;
+      if (k == 0) f_lower = f;
+      else if (k == 1) f_upper = f;
+      else f_min = f;
+    }
     if (x_lower > x_upper || x_min >= x_upper || x_min <= x_lower) return ST_GSL;
     if (f_min >= f_lower || f_min >= f_upper) return ST_GSL;
     const double golden = 0.3819660;

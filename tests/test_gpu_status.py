@@ -1,0 +1,43 @@
+"""GPU parity of a query failure the reference raises mid-optimisation.
+
+In `Three_tax.optimize` the distal length search sets the proximal length to
+(dist + prox) - v and asserts that it is not negative (three_tax.ml:40-44).
+Rounding can make a probe at the top of the bracket trip that assertion; the
+reference then fails the query (`Child_error`, pplacer_run.ml:54) and the
+GPU reports status 6.  Among 2,000 queries of the 50-taxon workload with seed
+21, the oracle fails exactly queries 1171, 1387, 1721, 1802 and 1810 this
+way; the GPU must fail the same queries with the same status, and place the
+others.
+"""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FAILING = [1171, 1387, 1721, 1802, 1810]
+
+
+def _print_case(name):
+    print("\n[gpu case] %s" % name, flush=True)
+
+
+def test_prox_assertion_queries_fail_as_in_the_oracle(oracle):
+    from pplacer_amd import placer, synth
+    from pplacer_amd.placer import PP_FP64
+    from tests.helpers import oracle_setup
+    _print_case("workload gtr50 seed 21, 2000 queries")
+    wl = synth.make_workload("gtr50", seed=21, n_queries=2000)
+    model, pre, core, oq = oracle_setup(oracle, wl)
+    pl, queries, statd, rates = placer.setup_from_workload(wl)
+    try:
+        assert oq == queries
+        pick = FAILING + [0, 1, 1170, 1172, 1999]
+        want = [core.place(queries[i])[0] for i in pick]
+        assert want == [6] * len(FAILING) + [0] * 5
+        _print_case("gtr50 seed 21, all 2000 queries, fp64")
+        got, status, _ = pl.place(queries, precision=PP_FP64)
+        assert [int(status[i]) for i in pick] == want
+        assert [i for i in range(len(queries)) if status[i] != 0] == FAILING
+        assert all(got[i] == [] for i in FAILING)
+        assert all(len(got[i]) > 0 for i in range(len(queries)) if i not in FAILING)
+    finally:
+        pl.close()
