@@ -258,7 +258,10 @@ int pp_marginal_batch(pp_ctx *ctx, int n_jobs, const char *queries, const int *q
  * baseball with Brent at initial_tolerance, keep, refine at 1e-5,
  * ll_normalized_prob, with calc_pp the marginal likelihoods and posterior
  * probabilities (core.ml:378-403), and the ml_ratio sort.  out is [n_queries][keep_at_most],
- * n_out[q] placements each; status[q] is a PP_Q_* code. */
+ * n_out[q] placements each; status[q] is a PP_Q_* code.  In the 4-state FP32
+ * form, a query that fails on the prox_bl assertion (PP_Q_ASSERT) is placed
+ * again in the FP64 form, whose failures are the reference's.  Returns
+ * PP_EQUERY, with every output filled, when some query failed. */
 int pp_place_batch(pp_ctx *ctx, int n_queries, const char *queries, const pp_prefs *prefs,
                    int precision, pp_placement *out, int *n_out, int *status,
                    pp_stats *stats);

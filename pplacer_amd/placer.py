@@ -14,8 +14,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import refio
-from ._lib import (PP_FP32, PP_FP64, PP_PRIOR_FLAT_EXP, PP_PRIOR_INFORMATIVE, PP_PRIOR_UNIFORM,
-                   LaunchConfig, Placement, Prefs, Stats, check, lib)
+from ._lib import (PP_EQUERY, PP_FP32, PP_FP64, PP_PRIOR_FLAT_EXP, PP_PRIOR_INFORMATIVE,
+                   PP_PRIOR_UNIFORM, LaunchConfig, Placement, Prefs, Stats, check, lib)
 
 _dp = C.POINTER(C.c_double)
 
@@ -234,10 +234,13 @@ class Placer:
         n_out = np.zeros(max(Q, 1), np.int32)
         status = np.zeros(max(Q, 1), np.int32)
         stats = Stats()
-        self.upload_queries(queries)
-        self.place_resident(prefs, precision, stats)
-        check(lib().pp_results_download(self._ctx, out, _p(n_out, C.c_int), _p(status, C.c_int)),
-              "results_download")
+        # pp_place_batch: one pass, and in the FP32 form the FP64 retry of
+        # queries that failed on the prox_bl assertion; PP_EQUERY only says
+        # that some query failed, and status holds which
+        rc = lib().pp_place_batch(self._ctx, Q, "".join(queries).encode(), C.byref(prefs), precision,
+                                  out, _p(n_out, C.c_int), _p(status, C.c_int), C.byref(stats))
+        if rc != PP_EQUERY:
+            check(rc, "place_batch")
         res = []
         for q in range(Q):
             row = []

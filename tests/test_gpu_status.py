@@ -8,6 +8,12 @@ GPU reports status 6.  Among 2,000 queries of the 50-taxon workload with seed
 21, the oracle fails exactly queries 1171, 1387, 1721, 1802 and 1810 this
 way; the GPU must fail the same queries with the same status, and place the
 others.
+
+The FP32 form steers Brent with FP32 likelihoods, so the same drift can trip
+the assertion on other queries (119, 836, 870 and 1600 here).  The batch entry
+places such queries again in the FP64 form: the FP32 form then fails exactly
+the reference's 5, and its results for the 4 equal the FP64 form's bit for
+bit.
 """
 import pytest
 
@@ -20,9 +26,14 @@ def _print_case(name):
     print("\n[gpu case] %s" % name, flush=True)
 
 
+def _fields(rows):
+    return [(p.edge_num, p.likelihood, p.like_weight_ratio, p.distal_length, p.pendant_length)
+            for p in rows]
+
+
 def test_prox_assertion_queries_fail_as_in_the_oracle(oracle):
     from pplacer_amd import placer, synth
-    from pplacer_amd.placer import PP_FP64
+    from pplacer_amd.placer import PP_FP32, PP_FP64
     from tests.helpers import oracle_setup
     _print_case("workload gtr50 seed 21, 2000 queries")
     wl = synth.make_workload("gtr50", seed=21, n_queries=2000)
@@ -39,5 +50,11 @@ def test_prox_assertion_queries_fail_as_in_the_oracle(oracle):
         assert [i for i in range(len(queries)) if status[i] != 0] == FAILING
         assert all(got[i] == [] for i in FAILING)
         assert all(len(got[i]) > 0 for i in range(len(queries)) if i not in FAILING)
+        _print_case("gtr50 seed 21, all 2000 queries, fp32")
+        got32, status32, _ = pl.place(queries, precision=PP_FP32)
+        assert [i for i in range(len(queries)) if status32[i] != 0] == FAILING
+        assert all(int(status32[i]) == 6 for i in FAILING)
+        retried = [119, 836, 870, 1600]
+        assert [_fields(got32[i]) for i in retried] == [_fields(got[i]) for i in retried]
     finally:
         pl.close()
