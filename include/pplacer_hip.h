@@ -261,7 +261,10 @@ int pp_marginal_batch(pp_ctx *ctx, int n_jobs, const char *queries, const int *q
  * n_out[q] placements each; status[q] is a PP_Q_* code.  In the 4-state FP32
  * form, a query that fails on the prox_bl assertion (PP_Q_ASSERT) is placed
  * again in the FP64 form, whose failures are the reference's.  Returns
- * PP_EQUERY, with every output filled, when some query failed. */
+ * PP_EQUERY, with every output filled, when some query failed.  It replaces
+ * the resident queries, and after it returns they are unspecified (the
+ * retry uploads its own): use the pp_queries_upload / pp_place_resident /
+ * pp_results_download trio for a resident set. */
 int pp_place_batch(pp_ctx *ctx, int n_queries, const char *queries, const pp_prefs *prefs,
                    int precision, pp_placement *out, int *n_out, int *status,
                    pp_stats *stats);
