@@ -123,9 +123,11 @@ def cpu_baseline(wl, queries, n_sample, threads):
     dt1 = time.perf_counter() - t1
     return {"value": len(sample) * E / dt, "unit": "query x edge evals/s", "cores": threads,
             "kind": "port",
-            "sample": "%d queries of the same workload (FP64 oracle, %d threads, %.1f s wall, "
-                      "%.1f ms/query/thread)" % (len(sample), threads, dt,
-                                                  1000 * dt * threads / len(sample)),
+            "sample": "%d queries of the same workload (FP64 oracle, one query per worker as "
+                      "pplacer -j N, N = %d threads of the %d cores in this process's affinity "
+                      "mask, %.1f s wall, %.1f ms/query/thread)"
+                      % (len(sample), threads, len(os.sched_getaffinity(0)), dt,
+                         1000 * dt * threads / len(sample)),
             "queries_per_s": len(sample) / dt,
             "one_thread": {"value": len(one) * E / dt1, "queries_per_s": len(one) / dt1,
                            "sample": "%d queries, 1 thread, %.1f s" % (len(one), dt1)}}
@@ -264,7 +266,9 @@ def main():
         # launches of one phase run concurrently on several streams, so the
         # phase as a whole reaches phase_achieved
         achieved = st_main["pitch_u2_tflops_per_launch"]
-        roofline = {"bound": "mfma",
+        # bound by the vector ALU: the 4-state contractions are 4x4, below any
+        # MFMA tile, and the rows stay in registers (no HBM stream per probe)
+        roofline = {"bound": "valu",
                     "pipe": ("vector ALU %s (4-state contractions do not use MFMA)" % args.precision
                              if S == 4 else "vector ALU fp64 (eigenbasis evolves)"),
                     "kernel": name, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
@@ -279,7 +283,9 @@ def main():
             roofline["note"] = "the refinement kernels took longer than the baseball kernel"
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # pplacer -j N with N = the host cores this process may use, capped
+        # at the 16 cores a one-GPU share of the host holds
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         log("CPU baseline: %d queries on %d threads" % (args.cpu_sample, threads))
         cpu = cpu_baseline(wl, queries, args.cpu_sample, threads)
     out = {

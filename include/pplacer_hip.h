@@ -49,7 +49,9 @@ enum {
   PP_Q_FIND_START = 4,     /* Minimization.FindStartFailure */
   PP_Q_INVALID_START = 5,  /* Minimization.InvalidStartValues */
   PP_Q_ASSERT = 6,         /* assert(prox_bl >= 0.) three_tax.ml:43 */
-  PP_Q_RANK_EXHAUSTED = 7, /* internal: ranking window too short (host retries) */
+  PP_Q_RANK_EXHAUSTED = 7, /* query too long for the GPU path: more masked sites than a
+                              4-state stage-2 workgroup's LDS indexes (16 bytes per site
+                              of 160 KB, about 9,000); no counterpart in the reference */
   PP_Q_MAXITER = 8,        /* Minimization.ExceededMaxIter outside the optimiser (-p) */
   PP_Q_GSL = 9             /* an uncaught Gsl_exn (-p: EBADFUNC, EBADTOL) */
 };
@@ -92,6 +94,24 @@ int pp_ten_masked_logdot(const double *x, const double *y, const uint16_t *mask,
 /* linear_c.c:541-605 ten_bounded_logdot_c -> *out */
 int pp_ten_bounded_logdot(const double *x, const double *y, int first, int last, double *util,
                           int n_rates, int n_sites, int n_states, double *out);
+
+/* The mat_* stubs of the Price-CAT model's rate-free [site][state] GLVs
+ * (linear_c.c:148-340, bound in linear.ml:28-49): the same arithmetic with one
+ * rate and no -n log(n_rates) term. */
+/* linear_c.c:168-213 mat_log_like3_c -> *out */
+int pp_mat_log_like3(const double *statd, const double *x, const double *y, const double *z,
+                     int n_sites, int n_states, double *out);
+/* linear_c.c:215-228 mat_pairwise_prod_c */
+int pp_mat_pairwise_prod(double *dst, const double *x, const double *y, int n_sites, int n_states);
+/* linear_c.c:230-248 mat_statd_pairwise_prod_c */
+int pp_mat_statd_pairwise_prod(const double *statd, double *dst, const double *a, const double *b,
+                               int n_sites, int n_states);
+/* linear_c.c:250-293 mat_masked_logdot_c -> *out */
+int pp_mat_masked_logdot(const double *x, const double *y, const uint16_t *mask, int n_sites,
+                         int n_states, double *out);
+/* linear_c.c:295-340 mat_bounded_logdot_c -> *out */
+int pp_mat_bounded_logdot(const double *x, const double *y, int first, int last, int n_sites,
+                          int n_states, double *out);
 
 /* ======================================================================= */
 /* 2. batched placement                                                     */
@@ -189,7 +209,10 @@ typedef struct {
   double t_ranking_ms;   /* stage 1 + rank ("ranking" timing, core.ml:212) */
   double t_ml_ms;        /* stage 2 ("ML calculation", core.ml:322) */
   int64_t n_logdots;     /* (query, edge) stage-1 scores */
-  int64_t n_like_calls;  /* log_like3 evaluations (Three_tax.n_like_calls) */
+  int64_t n_like_calls;  /* Model.log_like3 calls as the reference makes them
+                            (Three_tax.n_like_calls), the GSL minimiser's set
+                            re-evaluations included; the probes actually run are
+                            counted by n_site_probes_* */
   int64_t n_pitched;     /* Brent-optimised (query, edge) pairs */
   /* per-kernel device time (HIP events on the library's stream) */
   double t_stage1_ms;    /* stage-1 scoring kernels (tile codes + scores) */
@@ -257,21 +280,23 @@ int pp_marginal_batch(pp_ctx *ctx, int n_jobs, const char *queries, const int *q
 /* Full Core.process_query for a batch of queries: stage 1, ranking,
  * baseball with Brent at initial_tolerance, keep, refine at 1e-5,
  * ll_normalized_prob, with calc_pp the marginal likelihoods and posterior
- * probabilities (core.ml:378-403), and the ml_ratio sort.  out is [n_queries][keep_at_most],
- * n_out[q] placements each; status[q] is a PP_Q_* code.  In the 4-state FP32
- * form, a query that fails on the prox_bl assertion (PP_Q_ASSERT) is placed
- * again in the FP64 form, whose failures are the reference's.  Returns
- * PP_EQUERY, with every output filled, when some query failed.  It replaces
- * the resident queries, and after it returns they are unspecified (the
- * retry uploads its own): use the pp_queries_upload / pp_place_resident /
- * pp_results_download trio for a resident set. */
+ * probabilities (core.ml:378-403), and the ml_ratio sort.  out is
+ * [n_queries][max(1, keep_at_most)], n_out[q] placements each; status[q] is a
+ * PP_Q_* code.  Returns PP_EQUERY, with every output filled, when some query
+ * failed.  Equivalent to pp_queries_upload + pp_place_resident +
+ * pp_results_download: the batch stays resident afterwards. */
 int pp_place_batch(pp_ctx *ctx, int n_queries, const char *queries, const pp_prefs *prefs,
                    int precision, pp_placement *out, int *n_out, int *status,
                    pp_stats *stats);
 
 /* Device-resident variant for throughput measurement and pipelines: the
- * queries are uploaded once (pp_queries_upload), placed with no host
- * transfer inside pp_place_resident, and fetched with pp_results_download. */
+ * queries are uploaded once (pp_queries_upload), placed by pp_place_resident
+ * and fetched with pp_results_download.  In the 4-state FP32 form, the
+ * queries of a pass that fail on the prox_bl assertion (PP_Q_ASSERT; FP32
+ * likelihoods steer Brent along other paths) are placed again in the FP64
+ * form inside pp_place_resident (one status download and a pass over those
+ * queries alone; its work and time are added to stats), so both forms fail
+ * exactly the queries the reference fails. */
 int pp_queries_upload(pp_ctx *ctx, int n_queries, const char *queries);
 int pp_place_resident(pp_ctx *ctx, const pp_prefs *prefs, int precision, pp_stats *stats);
 int pp_results_download(pp_ctx *ctx, pp_placement *out, int *n_out, int *status);

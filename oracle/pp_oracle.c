@@ -161,6 +161,21 @@ double or_mat_masked_logdot(const double *x, const double *y, const uint16_t *ma
   return ll_tot;
 }
 
+/* linear_c.c:295-340 */
+double or_mat_bounded_logdot(const double *x, const double *y, int first, int last, int n_states)
+{
+  double ll_tot = 0;
+  x += (long)first * n_states;
+  y += (long)first * n_states;
+  for (int site = 0; site < 1 + last - first; site++) {
+    double util = 0;
+    for (int s = 0; s < n_states; s++) util += x[s] * y[s];
+    x += n_states; y += n_states;
+    ll_tot += log(util);
+  }
+  return ll_tot;
+}
+
 /* linear_c.c:170-216 */
 double or_mat_log_like3(const double *statd, const double *x, const double *y,
                         const double *z, int n_sites, int n_states)
@@ -792,8 +807,92 @@ void or_prep_supernodes(const or_model *m, const or_tree *t, int n_sites,
 
 #define SQRT_DBL_EPSILON 1.4901161193847656e-08
 
+/* GSL 2.x gsl_min_fminimizer_set (f(lower), f(upper), f(minimum), then
+ * set_with_values' checks) followed by the Brent type's init (v = w = lower +
+ * golden (upper - lower), d = e = 0, f(v)).  Returns OR_OK, OR_GSL_EBADFUNC
+ * (a non-finite value, GSL's SAFE_FUNC_CALL) or OR_GSL_EINVAL. */
+int or_gsl_brent_set(or_gsl_brent *s, or_fun f, void *ctx, double x_minimum,
+                     double x_lower, double x_upper)
+{
+  double f_lower = f(x_lower, ctx); if (!isfinite(f_lower)) return OR_GSL_EBADFUNC;
+  double f_upper = f(x_upper, ctx); if (!isfinite(f_upper)) return OR_GSL_EBADFUNC;
+  double f_min = f(x_minimum, ctx); if (!isfinite(f_min)) return OR_GSL_EBADFUNC;
+  return or_gsl_brent_set_with_values(s, f, ctx, x_minimum, f_min, x_lower, f_lower, x_upper,
+                                      f_upper);
+}
+
+int or_gsl_brent_set_with_values(or_gsl_brent *s, or_fun f, void *ctx, double x_minimum,
+                                 double f_minimum, double x_lower, double f_lower,
+                                 double x_upper, double f_upper)
+{
+  if (x_lower > x_upper) return OR_GSL_EINVAL;
+  if (x_minimum >= x_upper || x_minimum <= x_lower) return OR_GSL_EINVAL;
+  s->x_lower = x_lower; s->x_upper = x_upper; s->x_minimum = x_minimum;
+  s->f_lower = f_lower; s->f_upper = f_upper; s->f_minimum = f_minimum;
+  if (f_minimum >= f_lower || f_minimum >= f_upper) return OR_GSL_EINVAL;
+  const double golden = 0.3819660;
+  s->v = x_lower + golden * (x_upper - x_lower);
+  s->w = s->v;
+  s->d = 0.; s->e = 0.;
+  const double f_vw = f(s->v, ctx);
+  if (!isfinite(f_vw)) return OR_GSL_EBADFUNC;
+  s->f_v = f_vw; s->f_w = f_vw;
+  return OR_OK;
+}
+
+/* One step of GSL's Brent minimiser (golden section 0.3819660, step
+ * tolerance sqrt(DBL_EPSILON) |z|, parabolic step when admissible): one
+ * evaluation of f, then the bracket and the (v, w, x) history move. */
+int or_gsl_brent_iterate(or_gsl_brent *s, or_fun f, void *ctx)
+{
+  const double golden = 0.3819660;
+  const double x_left = s->x_lower, x_right = s->x_upper, z = s->x_minimum, f_z = s->f_minimum;
+  double d = s->e, e = s->d, u, f_u;
+  double v = s->v, w = s->w, f_v = s->f_v, f_w = s->f_w;
+  const double w_lower = z - x_left, w_upper = x_right - z;
+  const double tol = SQRT_DBL_EPSILON * fabs(z);
+  double p = 0, q = 0, r = 0;
+  const double midpoint = 0.5 * (x_left + x_right);
+  if (fabs(e) > tol) {
+    r = (z - w) * (f_z - f_v);
+    q = (z - v) * (f_z - f_w);
+    p = (z - v) * q - (z - w) * r;
+    q = 2 * (q - r);
+    if (q > 0) p = -p; else q = -q;
+    r = e;
+    e = d;
+  }
+  if (fabs(p) < fabs(0.5 * q * r) && p < q * w_lower && p < q * w_upper) {
+    double t2 = 2 * tol;
+    d = p / q;
+    u = z + d;
+    if ((u - x_left) < t2 || (x_right - u) < t2) d = (z < midpoint) ? tol : -tol;
+  } else {
+    e = (z < midpoint) ? x_right - z : -(z - x_left);
+    d = golden * e;
+  }
+  if (fabs(d) >= tol) u = z + d;
+  else u = z + ((d > 0) ? tol : -tol);
+  s->e = e; s->d = d;
+  f_u = f(u, ctx);
+  if (!isfinite(f_u)) return OR_GSL_EBADFUNC;
+  if (f_u <= f_z) {
+    if (u < z) { s->x_upper = z; s->f_upper = f_z; }
+    else { s->x_lower = z; s->f_lower = f_z; }
+    s->v = w; s->f_v = f_w; s->w = z; s->f_w = f_z;
+    s->x_minimum = u; s->f_minimum = f_u;
+  } else {
+    if (u < z) { s->x_lower = u; s->f_lower = f_u; }
+    else { s->x_upper = u; s->f_upper = f_u; }
+    if (f_u <= f_w || w == z) { s->v = w; s->f_v = f_w; s->w = u; s->f_w = f_u; }
+    else if (f_u <= f_v || v == z || v == w) { s->v = u; s->f_v = f_u; }
+  }
+  return OR_OK;
+}
+
 /* minimization.ml:83-100 with bisection_start_finder (:11-41) and the GSL
- * fminimizer (set: f(lower), f(upper), f(minimum); init: f(v); iterate). */
+ * fminimizer (set: f(lower), f(upper), f(minimum); init: f(v); iterate
+ * until the bracket is narrower than tolerance, at most max_iters times). */
 int or_brent(or_fun f, void *ctx, double raw_start, double left, double right,
              double tolerance, int max_iters, double *result)
 {
@@ -815,65 +914,14 @@ int or_brent(or_fun f, void *ctx, double raw_start, double left, double right,
     prev = (prev + smaller) / 2.;
     iter++;
   }
-  /* gsl_min_fminimizer_set */
-  double x_lower = left, x_upper = right, x_min = start;
-  double f_lower = f(x_lower, ctx); if (!isfinite(f_lower)) return OR_GSL_EBADFUNC;
-  double f_upper = f(x_upper, ctx); if (!isfinite(f_upper)) return OR_GSL_EBADFUNC;
-  double f_min = f(x_min, ctx); if (!isfinite(f_min)) return OR_GSL_EBADFUNC;
-  if (x_lower > x_upper) return OR_GSL_EINVAL;
-  if (x_min >= x_upper || x_min <= x_lower) return OR_GSL_EINVAL;
-  if (f_min >= f_lower || f_min >= f_upper) return OR_GSL_EINVAL;
-  /* Brent init */
-  const double golden = 0.3819660;
-  double v = x_lower + golden * (x_upper - x_lower), w = v;
-  double sd = 0., se = 0.;   /* state d, e */
-  double f_v = f(v, ctx); if (!isfinite(f_v)) return OR_GSL_EBADFUNC;
-  double f_w = f_v;
+  or_gsl_brent s;
+  int st = or_gsl_brent_set(&s, f, ctx, start, left, right);
+  if (st) return st;
   for (int step = 1;; step++) {
     if (step > max_iters) return OR_EXCEEDED_MAX_ITER;
-    /* Brent iterate */
-    const double x_left = x_lower, x_right = x_upper, z = x_min, f_z = f_min;
-    double d = se, e = sd, u, f_u;
-    const double w_lower = z - x_left, w_upper = x_right - z;
-    const double tol = SQRT_DBL_EPSILON * fabs(z);
-    double p = 0, q = 0, r = 0;
-    const double midpoint = 0.5 * (x_left + x_right);
-    if (fabs(e) > tol) {
-      r = (z - w) * (f_z - f_v);
-      q = (z - v) * (f_z - f_w);
-      p = (z - v) * q - (z - w) * r;
-      q = 2 * (q - r);
-      if (q > 0) p = -p; else q = -q;
-      r = e;
-      e = d;
-    }
-    if (fabs(p) < fabs(0.5 * q * r) && p < q * w_lower && p < q * w_upper) {
-      double t2 = 2 * tol;
-      d = p / q;
-      u = z + d;
-      if ((u - x_left) < t2 || (x_right - u) < t2) d = (z < midpoint) ? tol : -tol;
-    } else {
-      e = (z < midpoint) ? x_right - z : -(z - x_left);
-      d = golden * e;
-    }
-    if (fabs(d) >= tol) u = z + d;
-    else u = z + ((d > 0) ? tol : -tol);
-    se = e; sd = d;
-    f_u = f(u, ctx);
-    if (!isfinite(f_u)) return OR_GSL_EBADFUNC;
-    if (f_u <= f_z) {
-      if (u < z) { x_upper = z; f_upper = f_z; }
-      else { x_lower = z; f_lower = f_z; }
-      v = w; f_v = f_w; w = z; f_w = f_z;
-      x_min = u; f_min = f_u;
-    } else {
-      if (u < z) { x_lower = u; f_lower = f_u; }
-      else { x_upper = u; f_upper = f_u; }
-      if (f_u <= f_w || w == z) { v = w; f_v = f_w; w = u; f_w = f_u; }
-      else if (f_u <= f_v || v == z || v == w) { v = u; f_v = f_u; }
-    }
-    (void)f_lower; (void)f_upper;
-    if (x_upper - x_lower < tolerance) { *result = x_min; return OR_OK; }
+    st = or_gsl_brent_iterate(&s, f, ctx);
+    if (st) return st;
+    if (s.x_upper - s.x_lower < tolerance) { *result = s.x_minimum; return OR_OK; }
   }
 }
 

@@ -50,6 +50,7 @@ double or_mat_masked_logdot(const double *x, const double *y, const uint16_t *ma
                             int n_sites, int n_states);
 double or_mat_log_like3(const double *statd, const double *x, const double *y,
                         const double *z, int n_sites, int n_states);
+double or_mat_bounded_logdot(const double *x, const double *y, int first, int last, int n_states);
 
 /* ---- gstar_support.ml --------------------------------------------------- */
 int or_get_twoexp(double x);
@@ -143,6 +144,18 @@ void or_prep_supernodes(const or_model *m, const or_tree *t, int n_sites,
 typedef double (*or_fun)(double x, void *ctx);
 int or_brent(or_fun f, void *ctx, double raw_start, double left, double right,
              double tolerance, int max_iters, double *result);
+/* The GSL minimiser's state (gsl_min_fminimizer + brent_state_t) and its
+ * set / iterate steps, exposed so that tests can drive the raw iteration. */
+typedef struct {
+  double x_lower, x_upper, x_minimum, f_lower, f_upper, f_minimum;
+  double v, w, d, e, f_v, f_w;
+} or_gsl_brent;
+int or_gsl_brent_set(or_gsl_brent *s, or_fun f, void *ctx, double x_minimum,
+                     double x_lower, double x_upper);
+int or_gsl_brent_set_with_values(or_gsl_brent *s, or_fun f, void *ctx, double x_minimum,
+                                 double f_minimum, double x_lower, double f_lower,
+                                 double x_upper, double f_upper);
+int or_gsl_brent_iterate(or_gsl_brent *s, or_fun f, void *ctx);
 
 /* ---- Integration (integration.ml:28: GSL 2.x gsl_integration_qng, the
  * QUADPACK QNG method) ------------------------------------------------------ */

@@ -67,6 +67,11 @@ class OrStats(C.Structure):
 OR_FUN = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
 
 
+class OrGslBrent(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("x_lower", "x_upper", "x_minimum", "f_lower", "f_upper",
+                                          "f_minimum", "v", "w", "d", "e", "f_v", "f_w")]
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB_PATH
@@ -92,6 +97,7 @@ def lib():
             "or_ten_bounded_logdot": (d, [_dp, _dp, i, i, _dp, i, i, i]),
             "or_mat_masked_logdot": (d, [_dp, _dp, _u16p, i, i]),
             "or_mat_log_like3": (d, [_dp, _dp, _dp, _dp, i, i]),
+            "or_mat_bounded_logdot": (d, [_dp, _dp, i, i, i]),
             "or_get_twoexp": (i, [d]),
             "or_of_twoexp": (d, [i]),
             "or_total_twoexp": (d, [_i64p, i]),
@@ -121,6 +127,8 @@ def lib():
             "or_prep_supernodes": (None, [C.POINTER(OrModel), C.POINTER(OrTree), i, _dp, _i64p,
                                           _dp, _i64p, _dp, _i64p]),
             "or_brent": (i, [OR_FUN, v, d, d, d, d, i, _dp]),
+            "or_gsl_brent_set": (i, [C.POINTER(OrGslBrent), OR_FUN, v, d, d, d]),
+            "or_gsl_brent_iterate": (i, [C.POINTER(OrGslBrent), OR_FUN, v]),
             "or_stage1_scores": (i, [C.POINTER(OrCore), C.c_char_p, _dp]),
             "or_rank": (i, [_dp, i, _ip]),
             "or_three_tax_optimize": (i, [C.POINTER(OrCore), C.c_char_p, i, d, d, d, _dp, _dp,
@@ -199,6 +207,17 @@ def mat_masked_logdot(x, y, mask):
     mask = np.ascontiguousarray(mask, dtype=np.uint16)
     N, S = x.shape
     return lib().or_mat_masked_logdot(P(x), P(y), P(mask, C.c_uint16), N, S)
+
+
+def mat_log_like3(statd, x, y, z):
+    x = f64(x)
+    N, S = x.shape
+    return lib().or_mat_log_like3(P(f64(statd)), P(x), P(f64(y)), P(f64(z)), N, S)
+
+
+def mat_bounded_logdot(x, y, first, last):
+    x = f64(x)
+    return lib().or_mat_bounded_logdot(P(x), P(f64(y)), first, last, x.shape[1])
 
 
 def ten_statd_pairwise_prod(statd, a, b):
@@ -569,6 +588,27 @@ def brent(f, raw_start, left, right, tol, max_iters=100):
     res = C.c_double()
     st = lib().or_brent(fn, None, raw_start, left, right, tol, max_iters, C.byref(res))
     return st, res.value, calls
+
+
+class GslBrent:
+    """The oracle's GSL Brent minimiser driven step by step, as a GSL program
+    drives gsl_min_fminimizer_set / gsl_min_fminimizer_iterate."""
+
+    def __init__(self, f):
+        self.points = []
+
+        def cb(x, _ctx):
+            self.points.append(x)
+            return f(x)
+
+        self._fn = OR_FUN(cb)
+        self.s = OrGslBrent()
+
+    def set(self, x_minimum, x_lower, x_upper):
+        return lib().or_gsl_brent_set(C.byref(self.s), self._fn, None, x_minimum, x_lower, x_upper)
+
+    def iterate(self):
+        return lib().or_gsl_brent_iterate(C.byref(self.s), self._fn, None)
 
 
 def qng(f, a, b, epsabs, epsrel):

@@ -75,6 +75,11 @@ SIGNATURES = {
     "pp_ten_statd_pairwise_prod": (_i, [_dp, _dp, _dp, _dp, _i, _i, _i]),
     "pp_ten_masked_logdot": (_i, [_dp, _dp, _u16p, _dp, _i, _i, _i, _dp]),
     "pp_ten_bounded_logdot": (_i, [_dp, _dp, _i, _i, _dp, _i, _i, _i, _dp]),
+    "pp_mat_log_like3": (_i, [_dp, _dp, _dp, _dp, _i, _i, _dp]),
+    "pp_mat_pairwise_prod": (_i, [_dp, _dp, _dp, _i, _i]),
+    "pp_mat_statd_pairwise_prod": (_i, [_dp, _dp, _dp, _dp, _i, _i]),
+    "pp_mat_masked_logdot": (_i, [_dp, _dp, C.POINTER(C.c_uint16), _i, _i, _dp]),
+    "pp_mat_bounded_logdot": (_i, [_dp, _dp, _i, _i, _i, _i, _dp]),
     "pp_ctx_create": (_i, [_i, C.POINTER(_vp)]),
     "pp_ctx_destroy": (_i, [_vp]),
     "pp_model_build": (_i, [_vp, _i, _i, _dp, _dp, _dp]),

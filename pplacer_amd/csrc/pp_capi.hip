@@ -45,7 +45,8 @@ namespace {
 
 constexpr int kTQ = 16;           // queries per stage-1 tile
 constexpr int kChunk = 32768;     // queries per device batch
-constexpr int kBuckets = 30;      // stage-2 masked-length classes of 64 sites
+constexpr int kLongBucket = 30;   // stage-2 queries beyond the register-resident classes
+constexpr int kBuckets = 31;      // stage-2 masked-length classes of 64 sites, then kLongBucket
 constexpr int kBucketStreams = PP_MAX_BUCKET_STREAMS;  // streams for a stage-2 phase's buckets
 static_assert(kChunk <= PP_MAX_CHUNK, "default chunk out of range");
 // Default streams a 4-state stage-2 phase spreads its buckets over (0: the
@@ -106,6 +107,8 @@ struct pp_ctx {
   DBuf b_de32, b_pe32;  // exponents of the FP32 copies
   DBuf b_lt;            // stage-1 log-util table [N][n_inf+1][Epad]
   DBuf b_scratch;       // 20-state stage-2 pair scratch
+  DBuf b_sscr;          // 4-state streaming-form product scratch [blocks][R*4][long_cap]
+  int sscr_blocks = 0;  // workgroups b_sscr holds
   DBuf b_cat;           // Price-CAT: category of each pre-masked site [N]
   // -p prior (Core.prior): kind, flat mean or per-edge means; device [G] means
   int prior_kind = PP_PRIOR_FLAT_EXP;
@@ -118,9 +121,15 @@ struct pp_ctx {
   std::vector<int> order;     // sorted position -> caller's index
   std::vector<int> qM, qlo, qhi, qstat;
   DBuf b_codes, b_qM;
+  // stage 2: longest masked length of the register-resident classes, and the
+  // longest resident query beyond it (0: none; those take the streaming form)
+  int reg_limit = 0, long_cap = 0;
   // work buffers
   DBuf b_tcodes, b_tile_lo, b_tile_hi, b_tile_q, b_ttab, b_scores, b_res, b_nres, b_status,
       b_kept, b_nkeep, b_out, b_nout, b_qlist, b_counter, b_stats;
+  // the FP64 re-placement of an FP32 pass's assertion failures: its query
+  // set and outputs while the pass's own are set aside
+  DBuf b_rcodes, b_rqM, b_rout, b_rnout, b_rstatus, b_ridx;
   // results staging (sorted order)
   std::vector<pp_placement> h_out;
   std::vector<int> h_nout, h_status;
@@ -536,13 +545,20 @@ int occupancy_grid(const void *kern, int bd, size_t lds, int n_cu, int n_items) 
 
 // Stage-2 kernels: one master wave + ceil(M / (64 spl)) worker waves.  Three
 // launch-bound classes (512, 768, 1024 lanes) bound the block size; the
-// kernels are built for 4 waves per SIMD (MWOcc).
-// LDS: header and tables, masked-site list, the pitch kernel's per-edge
-// flags, cached products of the worker lanes' sites.
+// kernels are built for 4 waves per SIMD (MWOcc).  spl == 0 is the streaming
+// form: 1024 lanes, any masked length up to stream_cap.
+// LDS: header and tables, per-site logs / list / pendant rows (up to the
+// launch's capacity), the pitch kernel's per-edge flags, and in the
+// register-resident forms the cached products of the worker lanes' sites.
 template <typename Real, int R>
-size_t stage2_lds(const pp_ctx *c, int bd, int taken_bytes, int spl) {
-  const MWLayout<Real, R> lay(c->model.n_cats > 0 ? c->model.n_cats : 1, c->model.n_sl());
-  return lay.align(lay.bytes(c->N) + (size_t)taken_bytes) + MWProducts<Real, R * 4>::bytes((bd - 64) * spl);
+size_t stage2_lds(const pp_ctx *c, int bd, size_t taken, int spl, int stream_cap) {
+  const int cap = spl == 0 ? stream_cap : (bd - 64) * spl;
+  const MWLayout<Real, R> lay(c->model.n_cats > 0 ? c->model.n_cats : 1, c->model.n_sl(), cap, spl == 0);
+  return lay.align(lay.align(lay.bytes()) + taken) + (spl ? MWProducts<Real, R * 4>::bytes(cap) : 0);
+}
+size_t stage2_lds_fp64(const pp_ctx *c, int bd, size_t taken, int spl, int stream_cap) {
+  return c->model.R == 4 ? stage2_lds<double, 4>(c, bd, taken, spl, stream_cap)
+                         : stage2_lds<double, 1>(c, bd, taken, spl, stream_cap);
 }
 
 // Whether the FP32 form runs stage 2 in FP32.  The 20-state engine evaluates
@@ -552,27 +568,76 @@ size_t stage2_lds(const pp_ctx *c, int bd, int taken_bytes, int spl) {
 // FP32 stage-1 table).
 bool stage2_fp32_ok(const pp_ctx *c) { return c->model.S == 4; }
 
-// Longest masked length whose stage-2 workgroup fits in LDS (FP64, the
-// larger form); longer queries get PP_Q_RANK_EXHAUSTED ("query too long").
-int max_masked_sites(const pp_ctx *c) {
-  if (c->model.S == 20) return c->N;  // pair state lives in global scratch
+// Longest masked length of the register-resident 4-state classes whose
+// workgroup fits in LDS (FP64, the larger form, with the pitch kernel's flags).
+int register_limit(const pp_ctx *c) {
   int m = 0;
-  for (int b = 0; b < kBuckets; b++) {
+  for (int b = 0; b < kLongBucket; b++) {
     const int spl = spl_for(true, b), bd = block_for(b, spl);
     if (bd > 1024) break;
-    const size_t lds = c->model.R == 4 ? stage2_lds<double, 4>(c, bd, c->E, spl)
-                                       : stage2_lds<double, 1>(c, bd, c->E, spl);
-    if (lds > c->max_lds) break;
+    if (stage2_lds_fp64(c, bd, taken_bytes(c->E), spl, 0) > c->max_lds) break;
     m = 64 * (b + 1);
   }
   return m;
 }
 
+// Longest masked length the GPU path places: N for 20 states (pair state in
+// global scratch); for 4 states the streaming form's, whose LDS holds 16
+// bytes per masked site (log, column, pendant row) besides the header and
+// flags, so about 9,000 sites in the 160 KB of a gfx950 workgroup.  Longer
+// queries get PP_Q_RANK_EXHAUSTED ("query too long for the GPU path").
+int max_masked_sites(const pp_ctx *c) {
+  if (c->model.S == 20) return c->N;
+  int lo = 0, hi = c->N;
+  while (lo < hi) {  // largest cap whose streaming workgroup fits
+    const int mid = lo + (hi - lo + 1) / 2;
+    if (stage2_lds_fp64(c, 1024, taken_bytes(c->E), 0, mid) <= c->max_lds) lo = mid;
+    else hi = mid - 1;
+  }
+  return std::max(lo, std::min(register_limit(c), c->N));
+}
+
+// Stage-2 class of a query of masked length M (4 states): its 64-site class
+// when the register-resident form holds it, else the streaming class.
+int stage2_bucket(const pp_ctx *c, int M) {
+  if (c->model.S == 20) return bucket_of(std::min(M, 64 * kLongBucket));
+  return M <= c->reg_limit ? bucket_of(M) : kLongBucket;
+}
+
+// Product scratch of the streaming form: one slice of long_cap sites per
+// workgroup the device can hold at once (1024-lane blocks: at most two per
+// CU); sized once before a pass, so that no launch finds it reallocated.
+int ensure_stream_scratch(pp_ctx *c) {
+  if (c->model.S != 4 || c->long_cap == 0) return PP_OK;
+  const int blocks = 2 * c->n_cu;
+  double *p;
+  ENSURE(c->b_sscr, (size_t)blocks * c->long_cap * c->model.R * 4, &p);
+  c->sscr_blocks = blocks;
+  return PP_OK;
+}
+
+// Grid of a 4-state stage-2 launch: the workgroups the device holds at once
+// (the work queue spreads the items over them); the streaming form also
+// hands each workgroup its product scratch slice.
+template <typename Args, int SPL>
+int stage2_grid(pp_ctx *c, const void *kern, int bd, size_t lds, int n_items, Args &a) {
+  int grid = occupancy_grid<double, 4>(kern, bd, lds, c->n_cu, n_items);
+  if constexpr (SPL == 0) {
+    grid = std::min(grid, c->sscr_blocks);
+    a.scratch = c->b_sscr.p;
+    a.cap_sites = c->long_cap;
+  } else {
+    a.cap_sites = 0;
+  }
+  return grid;
+}
+
 template <typename Real, int R, int MAXBD, int SPL, bool CAT>
 int launch_pitch_b(pp_ctx *c, PitchArgs a, int bd) {
-  const size_t lds = stage2_lds<Real, R>(c, bd, c->E, SPL);
+  const size_t lds = stage2_lds<Real, R>(c, bd, taken_bytes(c->E), SPL, c->long_cap);
   const void *k = reinterpret_cast<const void *>(&pitch_mw_kernel<Real, R, MAXBD, SPL, CAT>);
-  const int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.nq);
+  const int grid = stage2_grid<PitchArgs, SPL>(c, k, bd, lds, a.nq, a);
+  CHECK(grid > 0, PP_ESTATE, "streaming-form scratch not allocated");
   hipLaunchKernelGGL((pitch_mw_kernel<Real, R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
@@ -580,18 +645,21 @@ int launch_pitch_b(pp_ctx *c, PitchArgs a, int bd) {
 
 template <typename Real, int R, int MAXBD, int SPL, bool CAT>
 int launch_opt_b(pp_ctx *c, OptArgs a, int bd) {
-  const size_t lds = stage2_lds<Real, R>(c, bd, 0, SPL);
+  const size_t lds = stage2_lds<Real, R>(c, bd, 0, SPL, c->long_cap);
   const void *k = reinterpret_cast<const void *>(&optimize_mw_kernel<Real, R, MAXBD, SPL, CAT>);
-  const int grid = occupancy_grid<Real, R>(k, bd, lds, c->n_cu, a.n_jobs);
+  const int grid = stage2_grid<OptArgs, SPL>(c, k, bd, lds, a.n_jobs, a);
+  CHECK(grid > 0, PP_ESTATE, "streaming-form scratch not allocated");
   hipLaunchKernelGGL((optimize_mw_kernel<Real, R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
 
 // Bucket b's launch: FP64 with one site per lane up to 15 worker waves, two
-// beyond (blocks of more than 512 lanes); FP32 with two.
+// beyond (blocks of more than 512 lanes); FP32 with two; the long bucket in
+// the streaming form.
 template <typename Real, int R, bool CAT = false>
 int launch_pitch(pp_ctx *c, PitchArgs a, int b) {
+  if (b == kLongBucket) return launch_pitch_b<Real, R, 1024, 0, CAT>(c, a, 1024);
   const int spl = spl_for(sizeof(Real) == 8, b), bd = block_for(b, spl);
   if (spl == 1) {
     if constexpr (sizeof(Real) == 8) {
@@ -609,6 +677,7 @@ int launch_pitch(pp_ctx *c, PitchArgs a, int b) {
 
 template <typename Real, int R, bool CAT = false>
 int launch_opt(pp_ctx *c, OptArgs a, int b) {
+  if (b == kLongBucket) return launch_opt_b<Real, R, 1024, 0, CAT>(c, a, 1024);
   const int spl = spl_for(sizeof(Real) == 8, b), bd = block_for(b, spl);
   if (spl == 1) {
     if constexpr (sizeof(Real) == 8) {
@@ -626,9 +695,10 @@ int launch_opt(pp_ctx *c, OptArgs a, int b) {
 
 template <int R, int MAXBD, int SPL, bool CAT>
 int launch_marg_b(pp_ctx *c, OptArgs a, int bd) {
-  const size_t lds = stage2_lds<double, R>(c, bd, 0, SPL);
+  const size_t lds = stage2_lds<double, R>(c, bd, 0, SPL, c->long_cap);
   const void *k = reinterpret_cast<const void *>(&marginal_mw_kernel<R, MAXBD, SPL, CAT>);
-  const int grid = occupancy_grid<double, R>(k, bd, lds, c->n_cu, a.n_jobs);
+  const int grid = stage2_grid<OptArgs, SPL>(c, k, bd, lds, a.n_jobs, a);
+  CHECK(grid > 0, PP_ESTATE, "streaming-form scratch not allocated");
   hipLaunchKernelGGL((marginal_mw_kernel<R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
@@ -636,6 +706,7 @@ int launch_marg_b(pp_ctx *c, OptArgs a, int bd) {
 
 template <int R, bool CAT = false>
 int launch_marg(pp_ctx *c, OptArgs a, int b) {
+  if (b == kLongBucket) return launch_marg_b<R, 1024, 0, CAT>(c, a, 1024);
   const int spl = spl_for(true, b), bd = block_for(b, spl);
   if (spl == 1) {
     if (bd <= 512) return launch_marg_b<R, 512, 1, CAT>(c, a, bd);
@@ -670,7 +741,7 @@ int prot_cap_pairs(const pp_ctx *c) {
 
 template <typename Real, int R>
 int launch_pitch_prot(pp_ctx *c, PitchArgs a) {
-  const size_t lds = ProtLayout::bytes(c->N) + (size_t)c->E;
+  const size_t lds = ProtLayout::align(ProtLayout::bytes(c->N)) + taken_bytes(c->E);
   const void *k = reinterpret_cast<const void *>(&pitch_prot_kernel<Real, R>);
   const int grid = occupancy_grid<Real, R>(k, kProtBlock, lds, c->n_cu, a.nq);
   a.cap_pairs = prot_cap_pairs(c);
@@ -784,6 +855,8 @@ int queries_upload(pp_ctx *c, int n_queries, const char *queries) {
   std::vector<uint8_t> codes((size_t)n_queries * N);
   const int n_inf = c->model.n_inf;
   const int max_m = max_masked_sites(c);
+  c->reg_limit = c->model.S == 4 ? register_limit(c) : 0;
+  c->long_cap = 0;
   for (int q = 0; q < n_queries; q++) {
     const char *s = queries + (size_t)q * N;
     uint8_t *o = codes.data() + (size_t)q * N;
@@ -796,6 +869,7 @@ int queries_upload(pp_ctx *c, int n_queries, const char *queries) {
     }
     if (st[q] == PP_Q_OK && m == 0) st[q] = PP_Q_NO_INFORMATIVE;
     if (st[q] == PP_Q_OK && m > max_m) st[q] = PP_Q_RANK_EXHAUSTED;
+    if (st[q] == PP_Q_OK && c->model.S == 4 && m > c->reg_limit) c->long_cap = std::max(c->long_cap, m);
     lo[q] = m ? l : 0;
     hi[q] = m ? h : 0;
     M[q] = m;
@@ -849,13 +923,13 @@ struct LaunchStream {
   ~LaunchStream() { c->cur = c->stream; }
 };
 
-// A placement pass over the resident queries, in chunks.  Stage 1 of every
+// One placement pass over the resident queries, in chunks.  Stage 1 of every
 // chunk runs on its own stream into one of two score buffers, so it overlaps
 // stage 2 of the previous chunk; stage 2 (baseball, keep, refinement, -p,
 // finalize) runs chunk after chunk on the main stream; in the 4-state form a
 // phase's buckets can run concurrently on up to kBucketStreams streams (the
 // tails of their work queues overlap), joined before the next phase.
-int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats) {
+int place_pass(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats) {
   CHECK(prefs, PP_EINVAL, "prefs is null");
   CHECK(precision == PP_FP64 || precision == PP_FP32, PP_EINVAL, "precision must be PP_FP64 or PP_FP32");
   CHECK(prefs->keep_at_most >= 0 && prefs->keep_at_most <= 64, PP_EINVAL, "keep_at_most must be in 0..64");
@@ -892,6 +966,8 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   ENSURE(c->b_nkeep, chunk, &d_nkeep);
   ENSURE(c->b_counter, 128, &d_counter);
   ENSURE(c->b_stats, 6, &d_stats);
+  st = ensure_stream_scratch(c);
+  if (st) return st;
   HIPCHECK(hipMemsetAsync(d_stats, 0, 6 * sizeof(unsigned long long), c->stream));
   if (Q > 0) HIPCHECK(hipMemcpyAsync(d_status, c->qstat.data(), (size_t)Q * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemsetAsync(d_nout, 0, (size_t)std::max(Q, 1) * 4, c->stream));
@@ -936,7 +1012,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     stage1_tiles(c, q0, nq, t_lo, t_hi, t_q);
     std::vector<std::vector<int>> buckets(kBuckets);
     for (int l = 0; l < nq; l++)
-      if (c->qstat[q0 + l] == PP_Q_OK) buckets[bucket_of(c->qM[q0 + l])].push_back(l);
+      if (c->qstat[q0 + l] == PP_Q_OK) buckets[stage2_bucket(c, c->qM[q0 + l])].push_back(l);
     std::vector<int> &boff = boffs[ci];
     int n = 0;
     for (int b = 0; b < kBuckets; b++) {
@@ -1079,6 +1155,7 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
     ka.Q = nq;
     ka.cap = cap;
     ka.K = K;
+    ka.keep = prefs->keep_at_most;
     ka.res = d_res;
     ka.n_res = d_nres;
     ka.status = status;
@@ -1212,6 +1289,109 @@ int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *st
   return PP_OK;
 }
 
+// Adds the counters and phase times of a second pass to a pass's stats.
+void add_stats(pp_stats *a, const pp_stats &b) {
+  a->t_ranking_ms += b.t_ranking_ms;
+  a->t_ml_ms += b.t_ml_ms;
+  a->n_logdots += b.n_logdots;
+  a->n_like_calls += b.n_like_calls;
+  a->n_pitched += b.n_pitched;
+  a->t_stage1_ms += b.t_stage1_ms;
+  a->t_pitch_ms += b.t_pitch_ms;
+  a->t_refine_ms += b.t_refine_ms;
+  a->n_site_pairs += b.n_site_pairs;
+  a->n_site_probes_pend += b.n_site_probes_pend;
+  a->n_site_probes_dist += b.n_site_probes_dist;
+  a->n_launches += b.n_launches;
+  a->t_table_ms += b.t_table_ms;
+  a->n_site_probes_pend_pitch += b.n_site_probes_pend_pitch;
+  a->n_site_probes_dist_pitch += b.n_site_probes_dist_pitch;
+  a->n_pitch_launches += b.n_pitch_launches;
+  a->t_pp_ms += b.t_pp_ms;
+  a->t_pitch_kernel_ms += b.t_pitch_kernel_ms;
+}
+
+// Queries of an FP32 pass that failed on the prox_bl >= 0 assertion
+// (three_tax.ml:41-45) are placed again in the FP64 form: FP32 likelihoods
+// steer Brent along other paths, where the one-ulp drift of dist + prox can
+// trip the assertion on queries the reference places; the FP64 form fails
+// exactly the reference's.  idx lists them (sorted positions, ascending).
+// The resident set and the pass's outputs are set aside while a pass over
+// those queries alone runs, whose results are then scattered back.
+int replace_in_fp64(pp_ctx *c, const pp_prefs *prefs, const std::vector<int> &idx, pp_stats *stats) {
+  const int n = (int)idx.size(), N = c->N, K = c->K_last;
+  int *d_idx;
+  uint8_t *d_rc;
+  int *d_rqM;
+  ENSURE(c->b_ridx, n, &d_idx);
+  ENSURE(c->b_rcodes, (size_t)n * N, &d_rc);
+  ENSURE(c->b_rqM, n, &d_rqM);
+  HIPCHECK(hipMemcpy(d_idx, idx.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(gather_codes_kernel, dim3(std::min(2048, (int)(((size_t)n * N + 255) / 256))), dim3(256), 0,
+                     c->stream, reinterpret_cast<const uint8_t *>(c->b_codes.p), d_idx, n, N, d_rc);
+  HIPCHECK(hipGetLastError());
+  std::vector<int> sub_M(n), sub_lo(n), sub_hi(n);
+  for (int j = 0; j < n; j++) {
+    sub_M[j] = c->qM[idx[j]];
+    sub_lo[j] = c->qlo[idx[j]];
+    sub_hi[j] = c->qhi[idx[j]];
+  }
+  HIPCHECK(hipMemcpy(d_rqM, sub_M.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  // set the resident set and outputs aside (the positions stay sorted by
+  // first informative column, as stage-1 tiling expects)
+  const int Q = c->Q;
+  std::vector<int> order_s = std::move(c->order), qM_s = std::move(c->qM), qlo_s = std::move(c->qlo),
+                   qhi_s = std::move(c->qhi), qstat_s = std::move(c->qstat);
+  auto swap_bufs = [c]() {
+    std::swap(c->b_codes, c->b_rcodes);
+    std::swap(c->b_qM, c->b_rqM);
+    std::swap(c->b_out, c->b_rout);
+    std::swap(c->b_nout, c->b_rnout);
+    std::swap(c->b_status, c->b_rstatus);
+  };
+  swap_bufs();
+  c->Q = n;
+  c->order.resize(n);
+  std::iota(c->order.begin(), c->order.end(), 0);
+  c->qM = sub_M;
+  c->qlo = sub_lo;
+  c->qhi = sub_hi;
+  c->qstat.assign(n, PP_Q_OK);
+  pp_stats sub{};
+  int st = place_pass(c, prefs, PP_FP64, &sub);
+  swap_bufs();
+  c->Q = Q;
+  c->order = std::move(order_s);
+  c->qM = std::move(qM_s);
+  c->qlo = std::move(qlo_s);
+  c->qhi = std::move(qhi_s);
+  c->qstat = std::move(qstat_s);
+  c->K_last = K;
+  if (st) return st;
+  hipLaunchKernelGGL(scatter_results_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                     reinterpret_cast<const pp_placement *>(c->b_rout.p), reinterpret_cast<const int *>(c->b_rnout.p),
+                     reinterpret_cast<const int *>(c->b_rstatus.p), d_idx, n, K,
+                     reinterpret_cast<pp_placement *>(c->b_out.p), reinterpret_cast<int *>(c->b_nout.p),
+                     reinterpret_cast<int *>(c->b_status.p));
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  if (stats) add_stats(stats, sub);
+  return PP_OK;
+}
+
+// A placement pass over the resident queries; in the 4-state FP32 form the
+// queries failing the prox_bl assertion are then placed again in FP64.
+int place_resident(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats) {
+  int st = place_pass(c, prefs, precision, stats);
+  if (st || precision != PP_FP32 || !stage2_fp32_ok(c) || c->Q == 0) return st;
+  std::vector<int> hs(c->Q), idx;
+  HIPCHECK(hipMemcpy(hs.data(), c->b_status.p, (size_t)c->Q * 4, hipMemcpyDeviceToHost));
+  for (int i = 0; i < c->Q; i++)
+    if (hs[i] == PP_Q_ASSERT) idx.push_back(i);
+  if (idx.empty()) return PP_OK;
+  return replace_in_fp64(c, prefs, idx, stats);
+}
+
 int results_download(pp_ctx *c, pp_placement *out, int *n_out, int *status) {
   const int Q = c->Q, K = c->K_last;
   std::vector<pp_placement> h((size_t)Q * K);
@@ -1283,7 +1463,8 @@ int pp_ctx_destroy(pp_ctx *c) {
                   &c->b_pe, &c->b_s64, &c->b_se, &c->b_de32, &c->b_pe32, &c->b_lt, &c->b_scratch, &c->b_cat, &c->b_prior, &c->b_codes, &c->b_qM, &c->b_tcodes,
                   &c->b_tile_lo, &c->b_tile_hi, &c->b_tile_q, &c->b_ttab, &c->b_scores, &c->b_res,
                   &c->b_nres, &c->b_status, &c->b_kept, &c->b_nkeep, &c->b_out, &c->b_nout,
-                  &c->b_qlist, &c->b_counter, &c->b_stats};
+                  &c->b_qlist, &c->b_counter, &c->b_stats, &c->b_sscr, &c->b_rcodes, &c->b_rqM,
+                  &c->b_rout, &c->b_rnout, &c->b_rstatus, &c->b_ridx};
   for (DBuf *b : bufs)
     if (b->p) (void)hipFree(b->p);
   (void)hipStreamDestroy(c->stream);
@@ -1501,35 +1682,6 @@ int pp_place_batch(pp_ctx *c, int n_queries, const char *queries, const pp_prefs
   if (st) return st;
   st = pp_results_download(c, out, n_out, status);
   if (st) return st;
-  // The FP32 form steers Brent with FP32 likelihoods, so the one-ulp drift
-  // of dist + prox can trip the prox_bl >= 0 assertion (three_tax.ml:43) on
-  // queries the reference places.  Those queries are placed again in the
-  // FP64 form, which fails exactly the queries the reference fails.
-  if (precision == PP_FP32 && stage2_fp32_ok(c)) {
-    std::vector<int> redo;
-    for (int q = 0; q < n_queries; q++)
-      if (status[q] == PP_Q_ASSERT) redo.push_back(q);
-    if (!redo.empty()) {
-      const int N = c->N, K = std::max(1, prefs->keep_at_most), R = (int)redo.size();
-      std::string sub;
-      sub.reserve((size_t)R * N);
-      for (int q : redo) sub.append(queries + (size_t)q * N, N);
-      std::vector<pp_placement> o2((size_t)R * K);
-      std::vector<int> n2(R), s2(R);
-      st = pp_queries_upload(c, R, sub.data());
-      if (st) return st;
-      st = pp_place_resident(c, prefs, PP_FP64, nullptr);
-      if (st) return st;
-      st = pp_results_download(c, o2.data(), n2.data(), s2.data());
-      if (st) return st;
-      for (int i = 0; i < R; i++) {
-        const int q = redo[i];
-        n_out[q] = n2[i];
-        status[q] = s2[i];
-        std::memcpy(out + (size_t)q * K, o2.data() + (size_t)i * K, sizeof(pp_placement) * K);
-      }
-    }
-  }
   for (int q = 0; q < n_queries; q++)
     if (status[q] != PP_Q_OK) {
       set_error("query " + std::to_string(q) + " failed with status " + std::to_string(status[q]));
@@ -1616,7 +1768,7 @@ int pp_optimize_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
   HIPCHECK(hipMemset(d_stats, 0, 32));
   std::vector<std::vector<int>> buckets(kBuckets);
   for (int j = 0; j < n_jobs; j++)
-    if (init_st[j] == PP_Q_OK) buckets[bucket_of(c->qM[jq[j]])].push_back(j);
+    if (init_st[j] == PP_Q_OK) buckets[stage2_bucket(c, c->qM[jq[j]])].push_back(j);
   std::vector<int> flat;
   std::vector<int> boff(kBuckets + 1, 0);
   for (int b = 0; b < kBuckets; b++) {
@@ -1625,6 +1777,8 @@ int pp_optimize_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
   }
   boff[kBuckets] = (int)flat.size();
   int *d_list = s.up(flat.data(), flat.size());
+  st = ensure_stream_scratch(c);
+  if (st) return st;
   for (int b = 0; b < kBuckets; b++) {
     const int n = boff[b + 1] - boff[b];
     if (!n) continue;
@@ -1704,7 +1858,7 @@ int pp_marginal_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
   HIPCHECK(hipMemset(d_stats, 0, 32));
   std::vector<std::vector<int>> buckets(kBuckets);
   for (int j = 0; j < n_jobs; j++)
-    if (init_st[j] == PP_Q_OK) buckets[bucket_of(c->qM[jq[j]])].push_back(j);
+    if (init_st[j] == PP_Q_OK) buckets[stage2_bucket(c, c->qM[jq[j]])].push_back(j);
   std::vector<int> flat;
   std::vector<int> boff(kBuckets + 1, 0);
   for (int b = 0; b < kBuckets; b++) {
@@ -1713,6 +1867,8 @@ int pp_marginal_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
   }
   boff[kBuckets] = (int)flat.size();
   int *d_list = s.up(flat.data(), flat.size());
+  st = ensure_stream_scratch(c);
+  if (st) return st;
   for (int b = 0; b < kBuckets; b++) {
     const int n = boff[b + 1] - boff[b];
     if (!n) continue;
@@ -1804,7 +1960,7 @@ int pp_ten_statd_pairwise_prod(const double *statd, double *dst, const double *a
 
 static int twin_logdot(const double *statd, const double *x, const double *y, const double *z,
                        const uint16_t *mask, int first, int last, double *util, int R, int N, int S,
-                       double *out) {
+                       double *out, int rate_term = 1) {
   CHECK(x && y && out && R > 0 && N >= 0 && S > 0, PP_EINVAL, "bad arguments");
   CHECK(first >= 0 && last < N && first <= last + 1, PP_EINVAL, "bad site interval");
   const int n_used = last - first + 1;
@@ -1818,7 +1974,7 @@ static int twin_logdot(const double *statd, const double *x, const double *y, co
   if (n_used > 0)
     hipLaunchKernelGGL(util_kernel, dim3((n_used + 255) / 256), dim3(256), 0, 0, du, ds, dx, dy, dz, dm, first,
                        n_used, R, N, S);
-  hipLaunchKernelGGL(logsum_kernel, dim3(1), dim3(64), 0, 0, du, dm, first, n_used, R, dout);
+  hipLaunchKernelGGL(logsum_kernel, dim3(1), dim3(64), 0, 0, du, dm, first, n_used, R, rate_term, dout);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipMemcpy(out, dout, 8, hipMemcpyDeviceToHost));
   if (util && n_used > 0) HIPCHECK(hipMemcpy(util, du, (size_t)n_used * 8, hipMemcpyDeviceToHost));
@@ -1840,6 +1996,35 @@ int pp_ten_masked_logdot(const double *x, const double *y, const uint16_t *mask,
 int pp_ten_bounded_logdot(const double *x, const double *y, int first, int last, double *util,
                           int n_rates, int n_sites, int n_states, double *out) {
   return twin_logdot(nullptr, x, y, nullptr, nullptr, first, last, util, n_rates, n_sites, n_states, out);
+}
+
+// The mat_* stubs: the rate-free [site][state] GLVs of Gcat_model; the same
+// kernels with one rate and no n log R term.
+int pp_mat_log_like3(const double *statd, const double *x, const double *y, const double *z, int n_sites,
+                     int n_states, double *out) {
+  CHECK(statd && z, PP_EINVAL, "null argument");
+  return twin_logdot(statd, x, y, z, nullptr, 0, n_sites - 1, nullptr, 1, n_sites, n_states, out, 0);
+}
+
+int pp_mat_pairwise_prod(double *dst, const double *x, const double *y, int n_sites, int n_states) {
+  return twin_prod(dst, nullptr, x, y, 1, n_sites, n_states);
+}
+
+int pp_mat_statd_pairwise_prod(const double *statd, double *dst, const double *a, const double *b,
+                               int n_sites, int n_states) {
+  CHECK(statd, PP_EINVAL, "statd is null");
+  return twin_prod(dst, statd, a, b, 1, n_sites, n_states);
+}
+
+int pp_mat_masked_logdot(const double *x, const double *y, const uint16_t *mask, int n_sites, int n_states,
+                         double *out) {
+  CHECK(mask, PP_EINVAL, "mask is null");
+  return twin_logdot(nullptr, x, y, nullptr, mask, 0, n_sites - 1, nullptr, 1, n_sites, n_states, out, 0);
+}
+
+int pp_mat_bounded_logdot(const double *x, const double *y, int first, int last, int n_sites, int n_states,
+                          double *out) {
+  return twin_logdot(nullptr, x, y, nullptr, nullptr, first, last, nullptr, 1, n_sites, n_states, out, 0);
 }
 
 }  // extern "C"

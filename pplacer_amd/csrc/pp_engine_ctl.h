@@ -146,7 +146,10 @@ struct ThreeTaxControl {
   double c_pend, c_dist, c_prox;
   double esum, last_ll;
   bool have_last;
-  long long n_calls, n_dist_probes;
+  // n_calls: Model.log_like3 calls as the reference makes them
+  // (Three_tax.n_like_calls), the GSL set's re-evaluations included;
+  // n_probes / n_dist_probes: probes this engine actually ran
+  long long n_calls, n_probes, n_dist_probes;
 
   __device__ __forceinline__ D &self() { return *static_cast<D *>(this); }
 
@@ -220,10 +223,13 @@ struct ThreeTaxControl {
         if (prox == (k == 0 ? prox_l : k == 1 ? prox_r : prox_p)) {
           dist_bl = x;
           prox_bl = prox;
+          n_calls++;  // the reference's call, answered from the value at hand
         } else {
           f = fprobe(kind, x, err);
           if (err) return err;
         }
+      } else {
+        n_calls++;
       }
       if (!isfinite(f)) return ST_GSL_BADFUNC;
       if (k == 0) f_lower = f;

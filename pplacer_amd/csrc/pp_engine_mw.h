@@ -16,7 +16,11 @@
 //   worker: ..................   barrier A -> [site work, wave partial sum] -> barrier B
 //
 // With SPL sites per lane, worker lane j of wave w holds masked sites
-// 64*(w-1)+j + k*64*W for k < SPL.
+// 64*(w-1)+j + k*64*W for k < SPL.  Queries longer than the registers and LDS
+// of a workgroup hold take the streaming form (SPL = 0): a worker lane walks
+// its sites 64*(w-1)+j + k*64*W for every k, re-reads the rows from the
+// resident GLV arrays at each distal probe and keeps the cached products in a
+// per-workgroup global scratch, so any masked length fits.
 //
 // Control state and site data live in different waves, so the register
 // allocator sizes the kernel for max(control, data) rather than their sum;
@@ -26,9 +30,10 @@
 // over j of P[i][j]*src[j]; each site's util the sequential (rate, state) sum
 // of ((pi*prox)*dist)*pend; P entries the sequential sum over k of
 // (exp(rate*bl*l_k)*x[i][k])*xit[j][k].  The FP64 form rounds every product
-// and sum separately (-ffp-contract=off); the FP32 form uses FMAs.  Per-site
-// logs are summed per wave and then in wave order (the stubs sum them
-// sequentially in site order).
+// and sum separately (-ffp-contract=off) and sums the per-site logs as the
+// stub does, sequentially in masked-site order (linear_c.c:416-419): workers
+// leave each site's log in LDS and the master adds them in order.  The FP32
+// form uses FMAs and sums the logs (in FP64) per wave, then in wave order.
 #pragma once
 
 #include <climits>
@@ -55,36 +60,42 @@ struct MWShared {
 };
 
 // LDS of one workgroup: shared header, pendant rows [n_tsl][kMaxCodes][R*4],
-// P(prox) and P(dist) [n_sl][4][4], exponential staging [3][n_sl][4],
-// masked-site list.  Gamma model: n_tsl = 1, n_sl = R (rate r uses slice r);
-// Price-CAT: R = 1 and n_tsl = n_sl = n_cats (a site uses its category's).
+// P(prox) and P(dist) [n_sl][4][4], exponential staging [3][n_sl][4], then
+// per masked site (cap = the longest masked length the launch takes): the
+// site's log of a probe (FP64 form), its column (the masked-site list) and in
+// the streaming form its pendant row.  Gamma model:
+// n_tsl = 1, n_sl = R (rate r uses slice r); Price-CAT: R = 1 and
+// n_tsl = n_sl = n_cats (a site uses its category's).
 template <typename Real, int R>
 struct MWLayout {
   static constexpr int RS = R * 4;
   static constexpr size_t head = (sizeof(MWShared) + 15) & ~size_t(15);
-  int n_tsl, n_sl;
-  __host__ __device__ MWLayout(int n_tsl_, int n_sl_) : n_tsl(n_tsl_), n_sl(n_sl_) {}
+  static constexpr bool strict = sizeof(Real) == 8;  // site logs summed in site order
+  int n_tsl, n_sl, cap;
+  bool stream;
+  __host__ __device__ MWLayout(int n_tsl_, int n_sl_, int cap_, bool stream_)
+      : n_tsl(n_tsl_), n_sl(n_sl_), cap(cap_), stream(stream_) {}
   __host__ __device__ size_t tbytes() const { return sizeof(Real) * (size_t)n_tsl * kMaxCodes * RS; }
   __host__ __device__ size_t pbytes() const { return sizeof(Real) * (size_t)n_sl * 16; }
   __host__ __device__ size_t ebytes() const { return sizeof(double) * (size_t)3 * n_sl * 4; }
-  __host__ __device__ size_t bytes(int N) const {
-    return head + tbytes() + 2 * pbytes() + ebytes() + sizeof(int) * (size_t)N;
-  }
+  __host__ __device__ size_t off_slog() const { return head + tbytes() + 2 * pbytes() + ebytes(); }
+  __host__ __device__ size_t off_sidx() const { return off_slog() + (strict ? sizeof(double) * (size_t)cap : 0); }
+  __host__ __device__ size_t bytes() const { return off_sidx() + sizeof(int) * (size_t)cap * (stream ? 2 : 1); }
   __device__ Real *T(char *s) const { return reinterpret_cast<Real *>(s + head); }
   __device__ Real *Pp(char *s) const { return reinterpret_cast<Real *>(s + head + tbytes()); }
   __device__ Real *Pd(char *s) const { return reinterpret_cast<Real *>(s + head + tbytes() + pbytes()); }
   __device__ double *E(char *s) const {
     return reinterpret_cast<double *>(s + head + tbytes() + 2 * pbytes());
   }
-  __device__ int *sidx(char *s) const {
-    return reinterpret_cast<int *>(s + head + tbytes() + 2 * pbytes() + ebytes());
-  }
+  __device__ double *slog(char *s) const { return reinterpret_cast<double *>(s + off_slog()); }
+  __device__ int *sidx(char *s) const { return reinterpret_cast<int *>(s + off_sidx()); }
+  __device__ int *sslot(char *s) const { return reinterpret_cast<int *>(s + off_sidx()) + cap; }
   static __host__ __device__ size_t align(size_t off) { return (off + 15) & ~size_t(15); }
 };
 
 template <typename Real, int R>
-__device__ __forceinline__ MWLayout<Real, R> mw_layout(const DevModel *dm) {
-  return MWLayout<Real, R>(dm->cat ? dm->n_sl : 1, dm->n_sl);
+__device__ __forceinline__ MWLayout<Real, R> mw_layout(const DevModel *dm, int cap, bool stream) {
+  return MWLayout<Real, R>(dm->cat ? dm->n_sl : 1, dm->n_sl, cap, stream);
 }
 
 // Wave-uniform branch on the wave index (wave 0 = master).
@@ -100,31 +111,6 @@ __device__ __forceinline__ void mw_copy_model(MWShared *sh, const DevModel *dm) 
 // ---------------------------------------------------------------------------
 // worker waves
 // ---------------------------------------------------------------------------
-// Sum over the lane's active sites of log(util).  In FP64 the SPL logs are
-// taken as one: log(prod of frexp mantissas) + (sum of exponents) * ln 2 (the
-// mantissa product of SPL <= 8 sites stays >= 2^-8, so nothing underflows).
-template <typename Real, int SPL>
-__device__ __forceinline__ double site_log_sum(const Real (&util)[SPL], const bool (&active)[SPL]) {
-  if constexpr (std::is_same<Real, double>::value && SPL > 1) {
-    static_assert(SPL <= 8, "mantissa product must stay normal");
-    double m = 1.0;
-    int e = 0;
-#pragma unroll
-    for (int k = 0; k < SPL; k++)
-      if (active[k]) {
-        m *= __builtin_amdgcn_frexp_mant(util[k]);
-        e += __builtin_amdgcn_frexp_exp(util[k]);
-      }
-    return log(m) + (double)e * PP_LOG2;
-  } else {
-    double lsum = 0.0;
-#pragma unroll
-    for (int k = 0; k < SPL; k++)
-      if (active[k]) lsum += RealOps<Real>::logd(util[k]);
-    return lsum;
-  }
-}
-
 // The cached products of masked site i live in LDS in groups of four sites:
 // byte (i/4)*kGroupBytes + t*4*sizeof(Real) + (i%4)*sizeof(Real).  Each lane's
 // sixteen products are immediate offsets from one address, and the 32-byte pad
@@ -139,19 +125,58 @@ struct MWProducts {
   }
 };
 
+// One masked site of a CMD_PROBE_DIST: evolve the proximal and distal rows
+// by P(prox) and P(dist) (sequential sums over j), form each (rate, state)
+// product ((pi*prox_ev)*dist_ev), hand it to keep(t, value) and accumulate
+// util = sum over (rate, state) of product*pend in that order.  pp / pd are
+// the site's P slices (for Price-CAT its category's; else rate r's slice is
+// pp + 16 r), z its evolved pendant row.
+template <typename Real, int R, bool CAT, class Keep>
+__device__ __forceinline__ Real site_dist_util(const Real *pp, const Real *pd, const Real *z,
+                                               const double *statd, const Real (&pO)[R * 4],
+                                               const Real (&dO)[R * 4], Keep &&keep) {
+  using Ops = RealOps<Real>;
+  Real util = 0;
+#pragma unroll
+  for (int r = 0; r < R; r++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const Real *ppr = pp + (CAT ? 0 : r * 16) + i * 4, *pdr = pd + (CAT ? 0 : r * 16) + i * 4;
+      const Real pi = (Real)statd[i];
+      const int t = r * 4 + i;
+      Real xv = ppr[0] * pO[r * 4], yv = pdr[0] * dO[r * 4];
+#pragma unroll
+      for (int j = 1; j < 4; j++) {
+        xv = Ops::madd(xv, ppr[j], pO[r * 4 + j]);
+        yv = Ops::madd(yv, pdr[j], dO[r * 4 + j]);
+      }
+      const Real w = (pi * xv) * yv;
+      keep(t, w);
+      // sums start from their first product: 0 + p == p up to the sign of a
+      // zero, which no later operation can observe
+      util = t == 0 ? w * z[t] : Ops::madd(util, w, z[t]);
+    }
+  return util;
+}
+
 // CAT: Price-CAT, R == 1, site k uses P slice and pendant table sl[k] = its
 // category (site_cat); otherwise slice r for rate r and pendant table 0.
+// FP64 (strict): each active site's log goes to slog[site] for the master's
+// sequential sum; FP32: the lane's logs are summed and reduced per wave.
 template <typename Real, int R, int SPL, bool CAT>
 __device__ __forceinline__ void mw_worker(char *smem, const MWLayout<Real, R> &lay, char *wl, int N,
                                           const uint8_t *codes, const Real *darr, const Real *parr,
                                           const int *darr_e, const int *parr_e, const int *site_cat) {
-  using Ops = RealOps<Real>;
   constexpr int RS = R * 4;
+  constexpr bool strict = MWLayout<Real, R>::strict;
   static_assert(!CAT || R == 1, "Price-CAT GLVs have no rate axis");
+  static_assert(SPL >= 1, "register-resident form");
   using W = MWProducts<Real, RS>;
+  using Ops = RealOps<Real>;
   MWShared *sh = reinterpret_cast<MWShared *>(smem);
   const Real *sT = lay.T(smem), *sPp = lay.Pp(smem), *sPd = lay.Pd(smem);
   const int *sidx = lay.sidx(smem);
+  double *slog = lay.slog(smem);
   const int lane = threadIdx.x & 63;
   const int wv = wave_id() - 1;
   const int stride = (int)blockDim.x - 64;  // worker lanes
@@ -198,48 +223,121 @@ __device__ __forceinline__ void mw_worker(char *smem, const MWLayout<Real, R> &l
       __syncthreads();  // A
       op = sh->op;
       while (op == CMD_PROBE || op == CMD_PROBE_DIST) {
-        // Sums start from their first product: 0 + p == p up to the sign of
-        // a zero, which no later operation can observe.
         Real util[SPL];
         if (op == CMD_PROBE_DIST) {
-          // util accumulates as each product is formed, in (rate, state) order
 #pragma unroll
-          for (int r = 0; r < R; r++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-              const Real *pp = sPp + r * 16 + i * 4, *pd = sPd + r * 16 + i * 4;
-              const Real pi = (Real)sh->statd[i];
-              const int t = r * 4 + i;
-#pragma unroll
-              for (int k = 0; k < SPL; k++) {
-                if (CAT) {
-                  pp = sPp + sl[k] * 16 + i * 4;
-                  pd = sPd + sl[k] * 16 + i * 4;
-                }
-                Real xv = pp[0] * pO[k][r * 4], yv = pd[0] * dO[k][r * 4];
-#pragma unroll
-                for (int j = 1; j < 4; j++) {
-                  xv = Ops::madd(xv, pp[j], pO[k][r * 4 + j]);
-                  yv = Ops::madd(yv, pd[j], dO[k][r * 4 + j]);
-                }
-                const Real wv_ = (pi * xv) * yv;
-                W::at(wl, wb[k], t) = wv_;
-                const Real z = sT[slot[k] * RS + t];
-                util[k] = t == 0 ? wv_ * z : Ops::madd(util[k], wv_, z);
-              }
-            }
+          for (int k = 0; k < SPL; k++) {
+            const int o = CAT ? sl[k] * 16 : 0;
+            util[k] = site_dist_util<Real, R, CAT>(sPp + o, sPd + o, sT + slot[k] * RS, sh->statd, pO[k], dO[k],
+                                                   [&](int t, Real w) { W::at(wl, wb[k], t) = w; });
+          }
         } else {
 #pragma unroll
           for (int t = 0; t < RS; t++)
 #pragma unroll
             for (int k = 0; k < SPL; k++) {
-              const Real wv_ = W::at(wl, wb[k], t), z = sT[slot[k] * RS + t];
-              util[k] = t == 0 ? wv_ * z : Ops::madd(util[k], wv_, z);
+              const Real w = W::at(wl, wb[k], t), z = sT[slot[k] * RS + t];
+              util[k] = t == 0 ? w * z : Ops::madd(util[k], w, z);
             }
         }
-        double lsum = site_log_sum<Real, SPL>(util, active);
-        lsum = wave_sum(lsum);
-        if (lane == 0) sh->part[wv] = lsum;
+        if constexpr (strict) {
+#pragma unroll
+          for (int k = 0; k < SPL; k++)
+            if (active[k]) slog[my + k * stride] = Ops::logd(util[k]);
+        } else {
+          double lsum = 0.0;
+#pragma unroll
+          for (int k = 0; k < SPL; k++)
+            if (active[k]) lsum += Ops::logd(util[k]);
+          lsum = wave_sum(lsum);
+          if (lane == 0) sh->part[wv] = lsum;
+        }
+        __syncthreads();  // B
+        __syncthreads();  // A
+        op = sh->op;
+      }
+    }
+  }
+  // op == CMD_EXIT
+}
+
+// Streaming form of the worker for masked lengths beyond the register-resident
+// form: lane j of worker wave w walks sites i = 64 (w-1) + j + k W64 for every
+// k while i < M.  The distal and proximal rows are read from darr / parr at
+// each distal probe (one 16-value row each), and the products cached for the
+// pendant-only probes live in prod[t * cap + i] (per-workgroup global scratch,
+// consecutive sites in consecutive addresses).  Same arithmetic as mw_worker.
+template <typename Real, int R, bool CAT>
+__device__ __forceinline__ void mw_worker_stream(char *smem, const MWLayout<Real, R> &lay, Real *prod, int cap,
+                                                 int N, const uint8_t *codes, const Real *darr,
+                                                 const Real *parr, const int *darr_e, const int *parr_e,
+                                                 const int *site_cat) {
+  constexpr int RS = R * 4;
+  constexpr bool strict = MWLayout<Real, R>::strict;
+  static_assert(!CAT || R == 1, "Price-CAT GLVs have no rate axis");
+  using Ops = RealOps<Real>;
+  MWShared *sh = reinterpret_cast<MWShared *>(smem);
+  const Real *sT = lay.T(smem), *sPp = lay.Pp(smem), *sPd = lay.Pd(smem);
+  const int *sidx = lay.sidx(smem);
+  int *sslot = lay.sslot(smem);
+  double *slog = lay.slog(smem);
+  const int lane = threadIdx.x & 63;
+  const int wv = wave_id() - 1;
+  const int stride = (int)blockDim.x - 64;
+  const int my = wv * 64 + lane;
+  __syncthreads();  // A
+  int op = sh->op;
+  while (op == CMD_QUERY) {
+    const int M = sh->M, q = sh->q;
+    for (int i = my; i < M; i += stride) {
+      const int site = sidx[i];
+      const int sl = CAT ? site_cat[site] : 0;
+      sslot[i] = sh->code_slot[codes[(size_t)q * N + site]] + sl * kMaxCodes;
+    }
+    __syncthreads();  // B
+    __syncthreads();  // A
+    op = sh->op;
+    while (op == CMD_GATHER) {
+      const size_t rowbase = (size_t)sh->loc * N;
+      long long es = 0;
+      for (int i = my; i < M; i += stride) {
+        const size_t row = rowbase + sidx[i];
+        es += (long long)darr_e[row] + (long long)parr_e[row];
+      }
+      es = wave_sum_ll(es);
+      if (lane == 0) sh->ipart[wv] = es;
+      __syncthreads();  // B
+      __syncthreads();  // A
+      op = sh->op;
+      while (op == CMD_PROBE || op == CMD_PROBE_DIST) {
+        double lsum = 0.0;
+        for (int i = my; i < M; i += stride) {
+          const int slot = sslot[i];
+          const Real *z = sT + slot * RS;
+          Real util;
+          if (op == CMD_PROBE_DIST) {
+            const int site = sidx[i];
+            const size_t row = rowbase + site;
+            Real dO[RS], pO[RS];
+#pragma unroll
+            for (int t = 0; t < RS; t++) { dO[t] = darr[row * RS + t]; pO[t] = parr[row * RS + t]; }
+            const int o = CAT ? (slot / kMaxCodes) * 16 : 0;
+            util = site_dist_util<Real, R, CAT>(sPp + o, sPd + o, z, sh->statd, pO, dO,
+                                                [&](int t, Real w) { prod[(size_t)t * cap + i] = w; });
+          } else {
+#pragma unroll
+            for (int t = 0; t < RS; t++) {
+              const Real w = prod[(size_t)t * cap + i];
+              util = t == 0 ? w * z[t] : Ops::madd(util, w, z[t]);
+            }
+          }
+          if constexpr (strict) slog[i] = Ops::logd(util);
+          else lsum += Ops::logd(util);
+        }
+        if constexpr (!strict) {
+          lsum = wave_sum(lsum);
+          if (lane == 0) sh->part[wv] = lsum;
+        }
         __syncthreads();  // B
         __syncthreads();  // A
         op = sh->op;
@@ -274,28 +372,31 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT>> {
   using Ctl::last_ll;
   using Ctl::have_last;
   using Ctl::n_calls;
+  using Ctl::n_probes;
   using Ctl::n_dist_probes;
   using L = MWLayout<Real, R>;
   static constexpr int RS = R * 4;
   MWShared *sh;
   Real *sT, *sPp, *sPd;
-  double *sE;
+  double *sE, *slog;
   int *sidx;
   int lane, n_workers, cap_sites, n_codes, n_inf, n_sl;
   double log_R;
 
-  __device__ __forceinline__ void init(char *smem, const DevModel *dm) {
-    const L lay = mw_layout<Real, R>(dm);
+  // cap: the longest masked length the launch takes (the layout's capacity)
+  __device__ __forceinline__ void init(char *smem, const DevModel *dm, int cap) {
+    const L lay = mw_layout<Real, R>(dm, cap, SPL == 0);
     sh = reinterpret_cast<MWShared *>(smem);
     sT = lay.T(smem);
     sPp = lay.Pp(smem);
     sPd = lay.Pd(smem);
     sE = lay.E(smem);
+    slog = lay.slog(smem);
     sidx = lay.sidx(smem);
     n_sl = dm->n_sl;
     lane = threadIdx.x & 63;
     n_workers = (blockDim.x >> 6) - 1;
-    cap_sites = 64 * SPL * n_workers;
+    cap_sites = cap;
     n_codes = dm->n_codes;
     n_inf = dm->n_inf;
     log_R = dm->log_R;
@@ -332,7 +433,10 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT>> {
     for (int m = 32; m >= 1; m >>= 1) present |= __shfl_xor(present, m, 64);
     int pos = incl - cnt;
     for (int k = lo; k < hi; k++)
-      if (codes[k] < n_inf) sidx[pos++] = k;
+      if (codes[k] < n_inf) {
+        if (pos < cap_sites) sidx[pos] = k;  // a longer query is refused below
+        pos++;
+      }
     if (lane == 0) {
       int ns = 4;
       for (int c = 0; c < n_codes; c++) {
@@ -473,23 +577,59 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT>> {
     }
   }
 
+  // ll_tot of ten_log_like3_c (linear_c.c:416-419): 0 + log(util[0]) +
+  // log(util[1]) + ... in masked-site order, one rounding per site, by lane 0
+  // from the logs the workers left in LDS; blocks of eight loads are issued
+  // ahead of their adds.
+  __device__ __forceinline__ double site_order_sum() {
+    double s = 0.0;
+    if (lane == 0) {
+      const int m = M;
+      int i = 0;
+      double v[8];  // slog[i .. i+8), entries at or beyond m unused
+#pragma unroll
+      for (int j = 0; j < 8; j++) v[j] = j < m ? slog[j] : 0.0;
+      for (; i + 16 <= m; i += 8) {
+        double nv[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) nv[j] = slog[i + 8 + j];
+#pragma unroll
+        for (int j = 0; j < 8; j++) s += v[j];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = nv[j];
+      }
+      // fewer than 16 left
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (i + j < m) s += v[j];
+      for (i += 8; i < m; i++) s += slog[i];
+    }
+    return readlane_f64(s, 0);
+  }
+
   // One likelihood probe (Model.log_like3 at the current three-taxon state).
   __device__ __forceinline__ double log_like() {
     const bool new_pend = !(pend_bl == c_pend);
     const bool new_dist = !(dist_bl == c_dist && prox_bl == c_prox);
     tables(new_pend, new_dist);
     command(new_dist ? CMD_PROBE_DIST : CMD_PROBE);
-    // partials of waves >= n_workers read as 0.0; the sum is in wave order
-    double pv[16];
+    double tot;
+    if constexpr (L::strict) {
+      tot = site_order_sum();
+    } else {
+      // partials of waves >= n_workers read as 0.0; the sum is in wave order
+      double pv[16];
 #pragma unroll
-    for (int w = 0; w < 16; w++) pv[w] = w < n_workers ? sh->part[w] : 0.0;
-    double tot = pv[0];
+      for (int w = 0; w < 16; w++) pv[w] = w < n_workers ? sh->part[w] : 0.0;
+      tot = pv[0];
 #pragma unroll
-    for (int w = 1; w < 16; w++) tot += pv[w];
+      for (int w = 1; w < 16; w++) tot += pv[w];
+    }
     c_pend = pend_bl;
     c_dist = dist_bl;
     c_prox = prox_bl;
     n_calls++;
+    n_probes++;
     n_dist_probes += new_dist;
     const double ll = (tot - (double)M * log_R) + PP_LOG2 * (esum + 0.0);
     last_ll = ll;

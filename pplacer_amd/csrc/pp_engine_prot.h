@@ -215,6 +215,7 @@ struct ProtMaster : ThreeTaxControl<ProtMaster<Real, R>> {
   using Ctl::last_ll;
   using Ctl::have_last;
   using Ctl::n_calls;
+  using Ctl::n_probes;
   using Ctl::n_dist_probes;
   static constexpr int S = kProtS;
   ProtShared *sh;
@@ -316,6 +317,7 @@ struct ProtMaster : ThreeTaxControl<ProtMaster<Real, R>> {
     c_dist = dist_bl;
     c_prox = prox_bl;
     n_calls++;
+    n_probes++;
     n_dist_probes += new_dist;
     const double ll = (tot - (double)M * log_R) + PP_LOG2 * (esum + 0.0);
     last_ll = ll;

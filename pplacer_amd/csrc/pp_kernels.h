@@ -53,17 +53,20 @@ struct PitchArgs {
   int cap;
   Res *res;  // [Q][cap]
   int *n_res, *status;
-  // n_calls[0]: probes; [2], [3]: sum of M over pendant-only / distal probes
+  // n_calls[0]: likelihood calls (Three_tax.n_like_calls); [2], [3]: sum of M
+  // over the pendant-only / distal probes actually run
   // (all stage-2 kernels); [4], [5]: the same for this kernel alone
   unsigned long long *n_calls, *n_pitched;
-  // 20-state engine: per-workgroup pair scratch (ProtLayout::scratch_reals(cap_pairs) each)
+  // 20-state engine: per-workgroup pair scratch (ProtLayout::scratch_reals(cap_pairs) each);
+  // 4-state streaming form: per-workgroup products [R*4][cap_sites]
   void *scratch;
-  int cap_pairs;
+  int cap_pairs, cap_sites;
   const int *site_cat;  // Price-CAT: category of each pre-masked site (else nullptr)
 };
 
 struct KeepArgs {
   int Q, cap, K;
+  int keep;  // keep_at_most (<= K, the stride of kept)
   const Res *res;
   const int *n_res;
   const int *status;
@@ -91,9 +94,9 @@ struct OptArgs {
   const double *job_pend, *job_dist;
   double *out_ll, *out_pend, *out_dist;
   int *out_calls, *out_status;
-  unsigned long long *n_calls;  // [0]: probes; [2], [3]: as in PitchArgs
-  void *scratch;                // 20-state engine, as in PitchArgs
-  int cap_pairs;
+  unsigned long long *n_calls;  // [0]: likelihood calls; [2], [3]: as in PitchArgs
+  void *scratch;                // as in PitchArgs
+  int cap_pairs, cap_sites;
   const int *site_cat;          // as in PitchArgs
   // marginal likelihoods (-p): mode 1 writes kept[].marg, mode 0 out_ll
   double pp_rel_err;

@@ -159,9 +159,14 @@ __device__ __forceinline__ void wave_argmin(unsigned long long &key, int &idx) {
   }
 }
 
+// Per-edge "already pitched" flags of the baseball loop, one bit per edge:
+// lane l owns edges e = l + 64 j, bit j % 32 of word taken[(j / 32) * 64 + l]
+// (a lane reads and writes only its own words).
+__host__ __device__ __forceinline__ size_t taken_bytes(int E) { return (size_t)256 * ((E + 2047) / 2048); }
+
 // Baseball over one work queue of queries, for either engine's master.
 template <class Master>
-__device__ __forceinline__ void pitch_loop(const PitchArgs &a, Master &m, uint8_t *taken) {
+__device__ __forceinline__ void pitch_loop(const PitchArgs &a, Master &m, unsigned *taken) {
   const int lane = m.lane;
   for (;;) {
     int qi = 0;
@@ -174,33 +179,41 @@ __device__ __forceinline__ void pitch_loop(const PitchArgs &a, Master &m, uint8_
       if (lane == 0) { a.n_res[q] = 0; a.status[q] = M == 0 ? PP_Q_NO_INFORMATIVE : PP_Q_RANK_EXHAUSTED; }
       continue;
     }
-    for (int e = lane; e < a.E; e += 64) taken[e] = 0;
+    const int n_words = (int)(taken_bytes(a.E) / 256);
+    for (int w = 0; w < n_words; w++) taken[w * 64 + lane] = 0u;
     const double *sc = a.scores + (size_t)q * a.E;
     double like_record = -INFINITY;
-    long long n_strikes = 0, calls = 0, dprobes = 0;
+    long long n_strikes = 0, calls = 0, probes = 0, dprobes = 0;
     int n_res = 0, status = PP_Q_OK;
     for (;;) {
-      // next edge of the stable descending sort of the scores (fig.ml:143-147)
+      // next edge of the stable descending sort of the scores (fig.ml:143-147;
+      // a nan score sorts last but is still pitched): bi == INT_MAX is the
+      // only "none left" marker
       unsigned long long bk = ~0ull;
       int bi = INT_MAX;
-      for (int e = lane; e < a.E; e += 64) {
-        if (taken[e]) continue;
+      for (int e = lane, j = 0; e < a.E; e += 64, j++) {
+        if ((taken[(j >> 5) * 64 + lane] >> (j & 31)) & 1u) continue;
         const unsigned long long k = desc_key(sc[e]);
-        if (k < bk) { bk = k; bi = e; }  // e increases: ties keep the smaller id
+        if (k < bk || (k == bk && e < bi)) { bk = k; bi = e; }
       }
       wave_argmin(bk, bi);
       bi = __builtin_amdgcn_readfirstlane(bi);
       if (bi == INT_MAX) break;
-      if (lane == (bi & 63)) taken[bi] = 1;
+      if (lane == (bi & 63)) {
+        const int j = bi >> 6;
+        taken[(j >> 5) * 64 + lane] |= 1u << (j & 31);
+      }
       const int loc = bi;
       m.gather(loc);
       const double cut = a.bl[loc];
       m.set_state(cut, a.start_pend, cut / 2.);
       m.n_calls = 0;
+      m.n_probes = 0;
       m.n_dist_probes = 0;
       double ll, pend, dist;
       const int st = m.safe_optimize(a.initial_tol, a.max_pend, a.start_pend, cut, &ll, &pend, &dist);
       calls += m.n_calls;
+      probes += m.n_probes;
       dprobes += m.n_dist_probes;
       if (is_gsl(st)) continue;  // "GSL problem with location": skipped (core.ml:286-297)
       if (st != ST_OK && st != ST_MAXITER) {
@@ -228,9 +241,9 @@ __device__ __forceinline__ void pitch_loop(const PitchArgs &a, Master &m, uint8_
       a.status[q] = (status == PP_Q_OK && n_res == 0) ? PP_Q_NO_RESULTS : status;
       atomicAdd(a.n_calls, (unsigned long long)calls);
       atomicAdd(a.n_pitched, (unsigned long long)n_res);
-      atomicAdd(a.n_calls + 2, (unsigned long long)((calls - dprobes) * M));
+      atomicAdd(a.n_calls + 2, (unsigned long long)((probes - dprobes) * M));
       atomicAdd(a.n_calls + 3, (unsigned long long)(dprobes * M));
-      atomicAdd(a.n_calls + 4, (unsigned long long)((calls - dprobes) * M));
+      atomicAdd(a.n_calls + 4, (unsigned long long)((probes - dprobes) * M));
       atomicAdd(a.n_calls + 5, (unsigned long long)(dprobes * M));
     }
   }
@@ -242,30 +255,55 @@ __device__ __forceinline__ void pitch_loop(const PitchArgs &a, Master &m, uint8_
 // sites per worker lane the rows fit the 128 VGPRs of 4 waves per SIMD, and
 // the master wave's control state spills only outside the per-probe loop.
 // Two FP64 sites per lane (the longest queries) keep the launch bound's budget.
+// SPL = 0 is the streaming form (any masked length, 1024-lane blocks).
 template <typename Real, int SPL>
 struct MWOcc {
   static constexpr int waves = (sizeof(Real) == 8 && SPL == 2) ? 1 : 4;
 };
 
+// The longest masked length a workgroup takes: its worker lanes times SPL,
+// or in the streaming form the capacity of its scratch slice.
+template <int SPL>
+__device__ __forceinline__ int mw_cap(int stream_cap) {
+  return SPL == 0 ? stream_cap : 64 * SPL * ((int)(blockDim.x >> 6) - 1);
+}
+
+// The worker waves of a 4-state stage-2 workgroup: register-resident rows
+// (SPL >= 1, cached products in LDS at wl) or the streaming form (SPL == 0,
+// products in the workgroup's slice of the global scratch).
+template <typename Real, int R, int SPL, bool CAT>
+__device__ __forceinline__ void mw_workers(char *smem, const MWLayout<Real, R> &lay, char *wl, const DevModel *,
+                                           int N, const uint8_t *codes, const void *darr, const void *parr,
+                                           const int *darr_e, const int *parr_e, const int *site_cat,
+                                           void *scratch, int cap) {
+  const Real *d = reinterpret_cast<const Real *>(darr);
+  const Real *p = reinterpret_cast<const Real *>(parr);
+  if constexpr (SPL == 0) {
+    Real *prod = reinterpret_cast<Real *>(scratch) + (size_t)blockIdx.x * cap * (R * 4);
+    mw_worker_stream<Real, R, CAT>(smem, lay, prod, cap, N, codes, d, p, darr_e, parr_e, site_cat);
+  } else {
+    mw_worker<Real, R, SPL, CAT>(smem, lay, wl, N, codes, d, p, darr_e, parr_e, site_cat);
+  }
+}
+
 template <typename Real, int R, int MAXBD, int SPL, bool CAT>
 __global__ __launch_bounds__(MAXBD) __attribute__((amdgpu_waves_per_eu(MWOcc<Real, SPL>::waves)))
 void pitch_mw_kernel(PitchArgs a) {
   extern __shared__ __align__(16) char smem[];
-  const MWLayout<Real, R> lay = mw_layout<Real, R>(a.model);
+  const int cap = mw_cap<SPL>(a.cap_sites);
+  const MWLayout<Real, R> lay = mw_layout<Real, R>(a.model, cap, SPL == 0);
   MWShared *sh = reinterpret_cast<MWShared *>(smem);
   mw_copy_model(sh, a.model);
   __syncthreads();
-  const Real *darr = reinterpret_cast<const Real *>(a.darr);
-  const Real *parr = reinterpret_cast<const Real *>(a.parr);
-  // taken[e]: read and written only by the lane that owns edge e (e % 64)
-  uint8_t *taken = reinterpret_cast<uint8_t *>(smem + lay.bytes(a.N));
+  unsigned *taken = reinterpret_cast<unsigned *>(smem + lay.align(lay.bytes()));
   if (wave_id() != 0) {
-    char *wl = smem + lay.align(lay.bytes(a.N) + a.E);
-    mw_worker<Real, R, SPL, CAT>(smem, lay, wl, a.N, a.codes, darr, parr, a.darr_e, a.parr_e, a.site_cat);
+    char *wl = smem + lay.align(lay.align(lay.bytes()) + taken_bytes(a.E));
+    mw_workers<Real, R, SPL, CAT>(smem, lay, wl, a.model, a.N, a.codes, a.darr, a.parr, a.darr_e, a.parr_e,
+                                  a.site_cat, a.scratch, a.cap_sites);
     return;
   }
   MWMaster<Real, R, SPL, CAT> m;
-  m.init(smem, a.model);
+  m.init(smem, a.model, cap);
   pitch_loop(a, m, taken);
 }
 
@@ -274,7 +312,7 @@ __global__ __launch_bounds__(512) void pitch_prot_kernel(PitchArgs a) {
   extern __shared__ __align__(16) char smem[];
   prot_copy_model(reinterpret_cast<ProtShared *>(smem), a.model);
   __syncthreads();
-  uint8_t *taken = reinterpret_cast<uint8_t *>(smem + ProtLayout::bytes(a.N));
+  unsigned *taken = reinterpret_cast<unsigned *>(smem + ProtLayout::align(ProtLayout::bytes(a.N)));
   if (wave_id() != 0) {
     Real *scr = reinterpret_cast<Real *>(a.scratch) + ProtLayout::scratch_reals(a.cap_pairs) * blockIdx.x;
     prot_worker<Real, R>(smem, scr, a.cap_pairs, a.N, a.codes, reinterpret_cast<const Real *>(a.darr),
@@ -323,12 +361,13 @@ __device__ __forceinline__ void optimize_loop(const OptArgs &a, Master &m) {
     const double cut = a.bl[loc];
     m.set_state(cut, pend0, dist0);
     m.n_calls = 0;
+    m.n_probes = 0;
     m.n_dist_probes = 0;
     double ll, pend, dist;
     const int st = m.safe_optimize(a.tol, a.max_pend, a.start_pend, cut, &ll, &pend, &dist);
     if (lane == 0) {
       atomicAdd(a.n_calls, (unsigned long long)m.n_calls);
-      atomicAdd(a.n_calls + 2, (unsigned long long)((m.n_calls - m.n_dist_probes) * M));
+      atomicAdd(a.n_calls + 2, (unsigned long long)((m.n_probes - m.n_dist_probes) * M));
       atomicAdd(a.n_calls + 3, (unsigned long long)(m.n_dist_probes * M));
       if (a.mode == 1) {
         if (st == ST_OK || st == ST_MAXITER) {
@@ -392,12 +431,13 @@ __device__ __forceinline__ void marginal_loop(const OptArgs &a, Master &m) {
     m.pr_kind = a.prior_kind;
     m.pr_mu = a.prior_mean[loc];
     m.n_calls = 0;
+    m.n_probes = 0;
     m.n_dist_probes = 0;
     double marg = 0.;
     const int st = m.calc_marg_prob(a.pp_rel_err, a.max_pend, &marg);
     if (lane == 0) {
       atomicAdd(a.n_calls, (unsigned long long)m.n_calls);
-      atomicAdd(a.n_calls + 2, (unsigned long long)((m.n_calls - m.n_dist_probes) * M));
+      atomicAdd(a.n_calls + 2, (unsigned long long)((m.n_probes - m.n_dist_probes) * M));
       atomicAdd(a.n_calls + 3, (unsigned long long)(m.n_dist_probes * M));
       if (a.mode == 1) {
         if (st == ST_OK) slot->marg = marg;
@@ -420,19 +460,19 @@ template <typename Real, int R, int MAXBD, int SPL, bool CAT>
 __global__ __launch_bounds__(MAXBD) __attribute__((amdgpu_waves_per_eu(MWOcc<Real, SPL>::waves)))
 void optimize_mw_kernel(OptArgs a) {
   extern __shared__ __align__(16) char smem[];
-  const MWLayout<Real, R> lay = mw_layout<Real, R>(a.model);
+  const int cap = mw_cap<SPL>(a.cap_sites);
+  const MWLayout<Real, R> lay = mw_layout<Real, R>(a.model, cap, SPL == 0);
   MWShared *sh = reinterpret_cast<MWShared *>(smem);
   mw_copy_model(sh, a.model);
   __syncthreads();
-  const Real *darr = reinterpret_cast<const Real *>(a.darr);
-  const Real *parr = reinterpret_cast<const Real *>(a.parr);
   if (wave_id() != 0) {
-    char *wl = smem + lay.align(lay.bytes(a.N));
-    mw_worker<Real, R, SPL, CAT>(smem, lay, wl, a.N, a.codes, darr, parr, a.darr_e, a.parr_e, a.site_cat);
+    char *wl = smem + lay.align(lay.bytes());
+    mw_workers<Real, R, SPL, CAT>(smem, lay, wl, a.model, a.N, a.codes, a.darr, a.parr, a.darr_e, a.parr_e,
+                                  a.site_cat, a.scratch, a.cap_sites);
     return;
   }
   MWMaster<Real, R, SPL, CAT> m;
-  m.init(smem, a.model);
+  m.init(smem, a.model, cap);
   optimize_loop(a, m);
 }
 
@@ -456,18 +496,18 @@ template <int R, int MAXBD, int SPL, bool CAT>
 __global__ __launch_bounds__(MAXBD) __attribute__((amdgpu_waves_per_eu(MWOcc<double, SPL>::waves)))
 void marginal_mw_kernel(OptArgs a) {
   extern __shared__ __align__(16) char smem[];
-  const MWLayout<double, R> lay = mw_layout<double, R>(a.model);
+  const int cap = mw_cap<SPL>(a.cap_sites);
+  const MWLayout<double, R> lay = mw_layout<double, R>(a.model, cap, SPL == 0);
   mw_copy_model(reinterpret_cast<MWShared *>(smem), a.model);
   __syncthreads();
   if (wave_id() != 0) {
-    char *wl = smem + lay.align(lay.bytes(a.N));
-    mw_worker<double, R, SPL, CAT>(smem, lay, wl, a.N, a.codes, reinterpret_cast<const double *>(a.darr),
-                                   reinterpret_cast<const double *>(a.parr), a.darr_e, a.parr_e,
-                                   a.site_cat);
+    char *wl = smem + lay.align(lay.bytes());
+    mw_workers<double, R, SPL, CAT>(smem, lay, wl, a.model, a.N, a.codes, a.darr, a.parr, a.darr_e, a.parr_e,
+                                    a.site_cat, a.scratch, a.cap_sites);
     return;
   }
   MWMaster<double, R, SPL, CAT> m;
-  m.init(smem, a.model);
+  m.init(smem, a.model, cap);
   marginal_loop(a, m);
 }
 
@@ -503,7 +543,7 @@ __global__ void keep_kernel(KeepArgs a) {
   unsigned long long used_mask[4] = {0, 0, 0, 0};  // up to 256 results tracked exactly
   double best = 0;
   int nk = 0;
-  for (int i = 0; i < a.K && i < n; i++) {
+  for (int i = 0; i < a.keep && i < n; i++) {
     unsigned long long bk = ~0ull;
     int bi = -1;
     for (int j = 0; j < n; j++) {
@@ -572,6 +612,27 @@ __global__ void finalize_kernel(FinalArgs a) {
     out[i].marginal_prob = a.calc_pp ? r.marg : __builtin_nan("");
   }
   a.n_out[q] = n;
+}
+
+// The character codes of queries idx[0 .. n) of a resident set, as a new
+// set of n queries (dst[n][N]).
+__global__ void gather_codes_kernel(const uint8_t *codes, const int *idx, int n, int N, uint8_t *dst) {
+  const size_t total = (size_t)n * N;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x)
+    dst[t] = codes[(size_t)idx[t / N] * N + t % N];
+}
+
+// The results of a pass over such a set back into the outputs of the pass
+// over the full set: query j of the set is query idx[j] there.
+__global__ void scatter_results_kernel(const pp_placement *src, const int *src_n, const int *src_st,
+                                       const int *idx, int n, int K, pp_placement *out, int *n_out,
+                                       int *status) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int q = idx[j];
+  for (int k = 0; k < K; k++) out[(size_t)q * K + k] = src[(size_t)j * K + k];
+  n_out[q] = src_n[j];
+  status[q] = src_st[j];
 }
 
 // Top-k of each query's ranking (pp_rank_topk), by the same selection the
@@ -654,15 +715,16 @@ __global__ void util_kernel(double *util, const double *statd, const double *x, 
   util[t] = u;
 }
 
-// sequential site-order log sum (linear_c.c:416-421 / 527-536)
+// sequential site-order log sum (linear_c.c:416-421 / 527-536); the mat_*
+// stubs (linear_c.c:170-340) have no n log R term (rate_term = 0)
 __global__ void logsum_kernel(const double *util, const uint16_t *mask, int first, int n_used,
-                              int R, double *out) {
+                              int R, int rate_term, double *out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double ll = 0;
   int used = 0;
   for (int t = 0; t < n_used; t++)
     if (!mask || mask[first + t]) { ll += log(util[t]); used++; }
-  ll -= ((double)(float)used) * log((double)(float)R);
+  if (rate_term) ll -= ((double)(float)used) * log((double)(float)R);
   *out = ll;
 }
 

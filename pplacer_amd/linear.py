@@ -108,3 +108,60 @@ def ten_bounded_logdot(x, y, first, last, util=None):
     if util is not None:
         util[: last - first + 1] = u[: last - first + 1]
     return out.value
+
+
+# ---- mat_* (the Price-CAT model's rate-free [site][state] GLVs) --------------
+
+def mat_log_like3(statd, x, y, z):
+    """linear_c.c:168-213 (no util vector, no n log R term)."""
+    statd, x, y, z = _f64(statd), _f64(x), _f64(y), _f64(z)
+    N, S = x.shape
+    if y.shape != x.shape or z.shape != x.shape:
+        raise ValueError("bad input dimensions")
+    out = C.c_double()
+    check(lib().pp_mat_log_like3(_p(statd), _p(x), _p(y), _p(z), N, S, C.byref(out)), "mat_log_like3")
+    return out.value
+
+
+def mat_pairwise_prod(dst, x, y):
+    x, y = _f64(x), _f64(y)
+    N, S = x.shape
+    out = np.empty_like(x)
+    check(lib().pp_mat_pairwise_prod(_p(out), _p(x), _p(y), N, S), "mat_pairwise_prod")
+    if dst is not None:
+        dst[...] = out
+    return out
+
+
+def mat_statd_pairwise_prod(statd, dst, a, b):
+    statd, a, b = _f64(statd), _f64(a), _f64(b)
+    N, S = a.shape
+    out = np.empty_like(a)
+    check(lib().pp_mat_statd_pairwise_prod(_p(statd), _p(out), _p(a), _p(b), N, S),
+          "mat_statd_pairwise_prod")
+    if dst is not None:
+        dst[...] = out
+    return out
+
+
+def mat_masked_logdot(x, y, mask):
+    """linear_c.c:250-293."""
+    x, y = _f64(x), _f64(y)
+    mask = np.ascontiguousarray(mask, dtype=np.uint16)
+    N, S = x.shape
+    if mask.shape[0] != N:
+        raise ValueError("mat_masked_logdot: Mask length doesn't match!")
+    out = C.c_double()
+    check(lib().pp_mat_masked_logdot(_p(x), _p(y), _p(mask, C.c_uint16), N, S, C.byref(out)),
+          "mat_masked_logdot")
+    return out.value
+
+
+def mat_bounded_logdot(x, y, first, last):
+    """linear_c.c:295-340."""
+    x, y = _f64(x), _f64(y)
+    N, S = x.shape
+    out = C.c_double()
+    check(lib().pp_mat_bounded_logdot(_p(x), _p(y), int(first), int(last), N, S, C.byref(out)),
+          "mat_bounded_logdot")
+    return out.value

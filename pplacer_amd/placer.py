@@ -295,6 +295,20 @@ class Placer:
               "place_resident")
         return stats
 
+    def download_results(self, n_queries: int, keep_at_most: int):
+        """pp_results_download after place_resident: (placements, status) as
+        place returns them, for the n_queries resident queries."""
+        K = max(1, keep_at_most)
+        out = (Placement * max(1, n_queries * K))()
+        n_out = np.zeros(max(n_queries, 1), np.int32)
+        status = np.zeros(max(n_queries, 1), np.int32)
+        check(lib().pp_results_download(self._ctx, out, _p(n_out, C.c_int), _p(status, C.c_int)),
+              "results_download")
+        res = [[PlacementRecord(out[q * K + i].loc, out[q * K + i].ml_ratio, out[q * K + i].log_like,
+                                out[q * K + i].dist_bl, out[q * K + i].pend_bl, None, None)
+                for i in range(n_out[q])] for q in range(n_queries)]
+        return res, status[:n_queries].copy()
+
     def synchronize(self):
         check(lib().pp_synchronize(self._ctx), "synchronize")
 

@@ -5,15 +5,28 @@
   a zero branch length (pplacer_src/newick_parser.mly:49-62,104-108,
   pplacer_src/newick_gtree.ml:145-157).
 * FASTA alignments, upper-cased (common_src/alignment.ml, ``upper_aln_of_any_file``).
+* Stockholm alignments (pplacer_src/stockholm.ml) and the suffix dispatch of
+  ``Alignment.list_of_any_file`` (common_src/alignment.ml:62-70).
 * refpkg ``phylo_model`` JSON for the gamma mixture model
-  (pplacer_src/gmix_model.ml:386-413).
+  (pplacer_src/gmix_model.ml:386-413) and the Price-CAT model.
+* reference packages: a directory or zip file with a CONTENTS.json naming the
+  tree, the alignment and the phylo_model (pplacer_src/refpkg_parse.ml:62-103,
+  pplacer_src/refpkg.ml:27-63).
+* the built-in WAG, JTT and LG protein models (pplacer_src/prot_models.ml:70-163),
+  shipped as numbers in data/prot_models.json.
 """
 from __future__ import annotations
 
+import gzip
 import json
+import os
+import re
+import zipfile
 from dataclasses import dataclass, field
 
 import numpy as np
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
 
 @dataclass
@@ -208,6 +221,76 @@ def read_fasta(path: str):
         return parse_fasta(f.read())
 
 
+# A Stockholm line is blank, the header, the footer, markup (#=...), or an
+# alignment row "name<blanks>sequence" (stockholm.ml:9-26).
+_STO_ROW = re.compile(r"^([^ \t\n\r#]+)[ \t]+([^ \t\n\r#]+)[ \t]*$")
+
+
+def parse_stockholm(text: str):
+    """Stockholm.parse (pplacer_src/stockholm.ml:36-82): the rows of each name
+    concatenated in file order, '.' read as '-', every sequence of one
+    length, and the records listed in descending name order (a fold over a
+    string map that conses each binding).  Sequences are upper-cased as
+    Alignment.upper_aln_of_any_file does."""
+    rows: dict = {}
+    state = "header"
+    for lineno, raw in enumerate(text.splitlines(), 1):
+        line = raw.rstrip(" \t\r")
+        if not line:
+            continue
+        if line == "# STOCKHOLM 1.0":
+            if state != "header":
+                raise ValueError("Stockholm line %d: header unexpected" % lineno)
+            state = "body"
+            continue
+        if state == "header":
+            raise ValueError("Stockholm line %d: something before header" % lineno)
+        if state == "done":
+            raise ValueError("Stockholm line %d: something after footer" % lineno)
+        if line == "//":
+            state = "done"
+            continue
+        if line.startswith("#="):
+            continue
+        m = _STO_ROW.match(line)
+        if not m:
+            raise ValueError("Stockholm line %d: not an alignment row: %r" % (lineno, line))
+        rows.setdefault(m.group(1), []).append(m.group(2).replace(".", "-"))
+    if state != "done":
+        raise ValueError("Stockholm: didn't reach footer by EOF")
+    out = [(name, "".join(parts).upper()) for name, parts in
+           sorted(rows.items(), key=lambda kv: kv[0].encode(), reverse=True)]
+    lengths = {len(seq) for _, seq in out}
+    if len(lengths) > 1:
+        raise ValueError("Stockholm: sequences of different lengths %s" % sorted(lengths))
+    return out
+
+
+def _read_text(path: str) -> str:
+    """A file's text, gunzipped when its name ends in .gz (MaybeZipped)."""
+    if path.endswith(".gz"):
+        with gzip.open(path, "rt") as f:
+            return f.read()
+    with open(path) as f:
+        return f.read()
+
+
+def parse_alignment(name: str, text: str):
+    """Alignment.list_of_any_file by the file name's suffix (a .gz suffix is
+    looked through): .fasta / .fa are FASTA, .sth / .sto Stockholm; records
+    upper-cased."""
+    base = name[:-3] if name.endswith(".gz") else name
+    if base.endswith((".fasta", ".fa")):
+        return parse_fasta(text)
+    if base.endswith((".sth", ".sto")):
+        return parse_stockholm(text)
+    raise ValueError("unfamiliar suffix on " + name)
+
+
+def read_alignment(path: str):
+    return parse_alignment(path, _read_text(path))
+
+
 def write_fasta(path: str, records) -> None:
     with open(path, "w") as f:
         for name, seq in records:
@@ -279,3 +362,100 @@ def exchangeabilities_from_lower(rows) -> np.ndarray:
             if i != j:
                 m[i, j] = rows[max(i, j) - 1][min(i, j)]
     return m
+
+
+# ---- built-in protein models (prot_models.ml:70-163) ------------------------
+
+def prot_model(name: str):
+    """Prot_models.trans_and_statd_of_model_name: (exchangeabilities [20][20],
+    model frequencies [20]) of LG, WAG or JTT."""
+    with open(os.path.join(DATA, "prot_models.json")) as f:
+        tables = json.load(f)
+    if name not in tables:
+        raise ValueError("unknown model: " + name)
+    t = tables[name]
+    return exchangeabilities_from_lower(t["lower"]), np.asarray(t["statd"], dtype=np.float64)
+
+
+def model_matrices(pm: PhyloModel, ref_seqs, emper_freq):
+    """Gstar_support.seqtype_and_trans_statd_of_info (gstar_support.ml:81-98):
+    (n_states, exchangeabilities, stationary frequencies) of a phylo_model;
+    emper_freq(n_states, seqs) gives the alignment's empirical frequencies.
+    GTR takes its subs_rates and always the alignment's frequencies; a protein
+    model its built-in table and, unless empirical_frequencies, its own
+    frequencies."""
+    if pm.subs_model == "GTR":
+        if pm.subs_rates is None:
+            raise ValueError("GTR specified but no substitution rates given.")
+        return 4, gtr_exchangeabilities(pm.subs_rates), emper_freq(4, ref_seqs)
+    exch, statd = prot_model(pm.subs_model)
+    return 20, exch, (emper_freq(20, ref_seqs) if pm.empirical_frequencies else statd)
+
+
+# ---- reference packages (refpkg_parse.ml, refpkg.ml) -------------------------
+
+class Refpkg:
+    """A reference package: a directory, or a zip file holding exactly one
+    directory, with a CONTENTS.json whose "files" object maps keys (tree,
+    aln_fasta, aln_sto, phylo_model, ...) to file names inside it
+    (Refpkg_parse.strmap_of_dir / strmap_of_zipfile).  Entries given in
+    `override` (key -> path) replace the package's, as pplacer's -t / -r do
+    (pplacer_run.ml:701-719)."""
+
+    def __init__(self, path: str | None = None, override: dict | None = None):
+        self._zip = None
+        self.files: dict = {}
+        self.name = None
+        self.format_version = None
+        if path is not None:
+            if os.path.isdir(path):
+                root = path.rstrip("/")
+                contents = json.load(open(os.path.join(root, "CONTENTS.json")))
+                self.files = {k: os.path.join(root, v) for k, v in contents["files"].items()}
+            else:
+                self._zip = zipfile.ZipFile(path)
+                dirs = {n.split("/")[0] + "/" for n in self._zip.namelist() if "/" in n}
+                if len(dirs) != 1:
+                    raise ValueError("reference package zipfiles must contain exactly one directory")
+                d = dirs.pop()
+                contents = json.loads(self._zip.read(d + "CONTENTS.json"))
+                self.files = {k: "zip:" + d + v for k, v in contents["files"].items()}
+            self.format_version = contents.get("metadata", {}).get("format_version")
+            base = os.path.basename(path.rstrip("/"))
+            self.name = base[:-len(".refpkg")] if base.endswith(".refpkg") else base
+        for k, v in (override or {}).items():
+            if v:
+                self.files[k] = v
+
+    def _text(self, key: str):
+        p = self.files[key]
+        if p.startswith("zip:"):
+            data = self._zip.read(p[4:])
+            if p.endswith(".gz"):
+                data = gzip.decompress(data)
+            return p[4:], data.decode()
+        return p, _read_text(p)
+
+    def has(self, key: str) -> bool:
+        return key in self.files
+
+    def tree(self) -> Tree:
+        """Refpkg.get_ref_tree."""
+        if "tree" not in self.files:
+            raise ValueError("the reference package provided does not contain a tree")
+        return parse_newick(self._text("tree")[1])
+
+    def alignment(self):
+        """Refpkg.get_aln_fasta: aln_fasta, else aln_sto; upper-cased."""
+        if "aln_fasta" in self.files:
+            return parse_fasta(self._text("aln_fasta")[1])
+        if "aln_sto" in self.files:
+            return parse_stockholm(self._text("aln_sto")[1])
+        raise ValueError("the reference package provided does not contain an alignment")
+
+    def phylo_model(self) -> PhyloModel:
+        """Refpkg.get_model's phylo_model (refpkg.ml:39-63)."""
+        if "phylo_model" not in self.files:
+            raise ValueError("the reference package provided does not contain a phylo_model "
+                             "(statistics files are not read)")
+        return parse_phylo_model(self._text("phylo_model")[1])

@@ -2,14 +2,17 @@
 (pplacer_src/pplacer_run.ml:167-783) for the gamma mixture and Price-CAT
 models, with the placement itself on the GPU.
 
-    python -m pplacer_amd.run --tree t.nwk --aln ref_and_queries.fasta \
-        --model phylo_model.json -o out.jplace [--fp32]
+    python -m pplacer_amd.run -c my.refpkg queries.fasta -o out.jplace [--fp32]
+    python -m pplacer_amd.run -t t.nwk -r ref.sto --model phylo_model.json queries.sto
 
-Reads the reference tree (edge numbers = pplacer's post-order ids), one
-alignment holding reference and query sequences (a sequence whose name is a
-leaf of the tree is a reference, pplacer_run.ml:658-680), and a refpkg
-phylo_model JSON (gmix_model.ml:386-413, gcat_model.ml:453-478).  Protein models need the
-exchangeability table passed as JSON (`--exch`, `{"lower": rows, "statd": [...]}`).
+The reference package (a directory or zip with CONTENTS.json) supplies the
+tree (edge numbers = pplacer's post-order ids), the reference alignment
+(aln_fasta or aln_sto) and the phylo_model; -t, -r and --model replace its
+entries.  The query file (FASTA or Stockholm, by suffix) holds the queries and,
+optionally, every reference sequence, which then replaces the package's
+alignment (pplacer_run.ml:658-680).  Protein models (LG, WAG, JTT) use the
+built-in exchangeabilities of prot_models.ml; --exch replaces them with a
+JSON table `{"lower": rows, "statd": [...]}`.
 """
 from __future__ import annotations
 
@@ -22,14 +25,36 @@ import numpy as np
 from . import jplace, placer, refio
 
 
-def partition(aln, tree: refio.Tree):
-    leaf_names = {tree.names[i] for i in tree.leaves()}
-    refs = [(n, s) for n, s in aln if n in leaf_names]
-    queries = [(n, s) for n, s in aln if n not in leaf_names]
-    missing = leaf_names - {n for n, _ in refs}
-    if missing:
-        raise ValueError("taxon not found in alignment: '%s'" % sorted(missing)[0])
-    return refs, queries
+def partition_queries(ref_names, aln):
+    """Pplacer_run.partition_queries (pplacer_run.ml:658-680): (queries,
+    reference rows found in the query file, or None when there are none)."""
+    refs = [(n, s) for n, s in aln if n in ref_names]
+    queries = [(n, s) for n, s in aln if n not in ref_names]
+    if not refs:
+        return queries, None
+    if len(refs) != len(ref_names):
+        raise ValueError("Some, but not all reference sequences found in provided alignment file. "
+                         "Are there overlaps between query and reference sequence IDs?")
+    seen = set()
+    for n, _ in refs:  # pplacer_run.ml:741-751
+        if n in seen:
+            raise ValueError("duplicate reference sequence '%s' in query file" % n)
+        seen.add(n)
+    return queries, refs
+
+
+def inputs_of_refpkg(rp: refio.Refpkg, query_aln):
+    """(tree, reference rows, queries, phylo_model) as run_file assembles them
+    (pplacer_run.ml:720-757): the tree's leaf names select the references, the
+    query file's reference rows replace the package's alignment."""
+    tree = rp.tree()
+    names = [tree.names[i] for i in tree.leaves()]
+    if len(set(names)) != len(names):
+        raise ValueError("Repeated names in reference tree!")
+    queries, refs = partition_queries(set(names), query_aln)
+    if refs is None:
+        refs = [(n, s) for n, s in rp.alignment() if n in set(names)]
+    return tree, refs, queries
 
 
 def cat_sites(site_categories, mask):
@@ -47,14 +72,11 @@ def cat_sites(site_categories, mask):
 def run_placements(tree: refio.Tree, refs, queries, pm: refio.PhyloModel, exch_table=None,
                    prefs=None, precision=placer.PP_FP64, device=0, prior="flat", prior_lower=0.0):
     """Returns (pqueries, status) where pqueries is [(namlom, [PlacementRecord])]."""
-    S = 4 if pm.subs_model == "GTR" else 20
     ref_seqs = [s for _, s in refs]
-    if S == 4:
-        exch = refio.gtr_exchangeabilities(pm.subs_rates)
-        statd = placer.emper_freq(4, ref_seqs)
+    if exch_table is None:
+        S, exch, statd = refio.model_matrices(pm, ref_seqs, placer.emper_freq)
     else:
-        if exch_table is None:
-            raise ValueError("protein model %s needs its exchangeability table" % pm.subs_model)
+        S = 20
         exch = refio.exchangeabilities_from_lower(exch_table["lower"])
         statd = (placer.emper_freq(20, ref_seqs) if pm.empirical_frequencies
                  else np.asarray(exch_table["statd"], dtype=float))
@@ -86,9 +108,15 @@ def run_placements(tree: refio.Tree, refs, queries, pm: refio.PhyloModel, exch_t
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--tree", required=True)
-    ap.add_argument("--aln", required=True)
-    ap.add_argument("--model", required=True)
+    ap.add_argument("query", nargs="?", default=None,
+                    help="query alignment (FASTA or Stockholm), optionally with the references")
+    ap.add_argument("-c", "--refpkg", default=None, help="reference package (directory or zip)")
+    ap.add_argument("-t", "--tree", default=None, help="reference tree (replaces the package's)")
+    ap.add_argument("-r", "--ref-aln", default=None,
+                    help="reference alignment (replaces the package's)")
+    ap.add_argument("--aln", default=None,
+                    help="one alignment with references and queries (instead of a query file)")
+    ap.add_argument("--model", default=None, help="phylo_model JSON (replaces the package's)")
     ap.add_argument("--exch", default=None)
     ap.add_argument("-o", "--out", required=True)
     ap.add_argument("--fp32", action="store_true")
@@ -115,9 +143,19 @@ def main(argv=None):
             args.keep_factor = 0.001
         if "--max-strikes" not in given:
             args.max_strikes = 20
-    tree = refio.read_newick(args.tree)
-    refs, queries = partition(refio.read_fasta(args.aln), tree)
-    pm = refio.read_phylo_model(args.model)
+    query_file = args.query or args.aln
+    if query_file is None:
+        ap.error("a query file is needed")
+    if args.refpkg is None and args.tree is None:
+        ap.error("please specify a reference tree with -t or -c")
+    ref_base = (args.ref_aln or "")[:-3] if (args.ref_aln or "").endswith(".gz") else (args.ref_aln or "")
+    key = "aln_sto" if ref_base.endswith((".sto", ".sth")) else "aln_fasta"
+    rp = refio.Refpkg(args.refpkg, {"tree": args.tree, "phylo_model": args.model,
+                                    key: args.ref_aln})
+    if args.ref_aln:  # -r replaces whichever alignment the package names
+        rp.files.pop("aln_fasta" if key == "aln_sto" else "aln_sto", None)
+    tree, refs, queries = inputs_of_refpkg(rp, refio.read_alignment(query_file))
+    pm = rp.phylo_model()
     exch_table = json.load(open(args.exch)) if args.exch else None
     prefs = placer.default_prefs(max_strikes=args.max_strikes, strike_box=args.strike_box,
                                  max_pitches=args.max_pitches, keep_at_most=args.keep_at_most,

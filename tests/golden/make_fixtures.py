@@ -7,7 +7,10 @@ Run once in a container where /root/reference exists:
   reference's tests/pplacer/test_like.ml:50-124 reads (PhyML-verified tree
   log-likelihoods are recorded in like/expected.json from test_like.ml:105-124).
 * prot_models.json: the WAG, JTT and LG exchangeabilities and frequencies, the
-  numeric tables of pplacer_src/prot_models.ml:70-163, as plain numbers.
+  numeric tables of pplacer_src/prot_models.ml:70-163, as plain numbers (also
+  written to pplacer_amd/data/, where they are the package's built-in models).
+* refpkg/: the reference's test refpkgs simple.refpkg and tiny.refpkg as they
+  are, and the donovan refpkg's CONTENTS.json.
 * price_cat/ar53_phylo_model.json: the Price-CAT phylo_model of the donovan
   regression refpkg (donovan-test/ar53_pplacer_testset/gtdb_r226_ar53.refpkg),
   the same JSON object written without indentation.
@@ -50,8 +53,11 @@ def prot_tables():
         rows = [[float(x) for x in row.split()] for row in re.findall(r'"([^"]*)"', m.group(1))]
         f = re.search(r'let %s_statd = parseFreq "([^"]*)"' % name, text)
         out[name.upper()] = {"lower": rows, "statd": [float(x) for x in f.group(1).split()]}
-    with open(os.path.join(HERE, "prot_models.json"), "w") as fh:
-        json.dump(out, fh)
+    # the fixture, and the same numbers as the package's built-in models
+    for path in (os.path.join(HERE, "prot_models.json"),
+                 os.path.join(HERE, "..", "..", "pplacer_amd", "data", "prot_models.json")):
+        with open(path, "w") as fh:
+            json.dump(out, fh)
 
 
 def copy_jplace():
@@ -75,8 +81,25 @@ def copy_price_cat():
         json.dump(obj, f, separators=(",", ":"))
 
 
+def copy_refpkgs():
+    """The reference's own test refpkgs (tests/data/{simple,tiny}.refpkg:
+    CONTENTS.json and the files it names) and the donovan refpkg's
+    CONTENTS.json (whose alignment is not in the checkout)."""
+    for name in ("simple.refpkg", "tiny.refpkg"):
+        src = os.path.join(REF, "tests/data", name)
+        dst = os.path.join(HERE, "refpkg", name)
+        os.makedirs(dst, exist_ok=True)
+        for n in os.listdir(src):
+            shutil.copy(os.path.join(src, n), os.path.join(dst, n))
+            os.chmod(os.path.join(dst, n), 0o644)
+    shutil.copy(os.path.join(REF, "donovan-test/ar53_pplacer_testset/gtdb_r226_ar53.refpkg/CONTENTS.json"),
+                os.path.join(HERE, "refpkg", "ar53_CONTENTS.json"))
+    os.chmod(os.path.join(HERE, "refpkg", "ar53_CONTENTS.json"), 0o644)
+
+
 if __name__ == "__main__":
     copy_like()
     prot_tables()
     copy_jplace()
     copy_price_cat()
+    copy_refpkgs()

@@ -59,6 +59,23 @@ def test_twins_match_oracle(oracle):
         u, lam, uit = rng.random((S, S)), rng.random(S), rng.random((S, S))
         np.testing.assert_array_equal(linear.dediagonalize(None, u, lam, uit),
                                       oracle.dediagonalize(u, lam, uit))
+    # the mat_* stubs (Price-CAT's rate-free GLVs), linear_c.c:148-340
+    for S in (4, 20):
+        N = 41
+        x, y, z = (rng.random((N, S)) for _ in range(3))
+        statd = rng.dirichlet(np.ones(S))
+        mask = (rng.random(N) < 0.6).astype(np.uint16)
+        assert linear.mat_log_like3(statd, x, y, z) == oracle.mat_log_like3(statd, x, y, z)
+        assert linear.mat_masked_logdot(x, y, mask) == oracle.mat_masked_logdot(x, y, mask)
+        assert linear.mat_bounded_logdot(x, y, 5, 30) == oracle.mat_bounded_logdot(x, y, 5, 30)
+        np.testing.assert_array_equal(linear.mat_pairwise_prod(None, x, y), x * y)
+        np.testing.assert_array_equal(linear.mat_statd_pairwise_prod(statd, None, x, y),
+                                      oracle.ten_statd_pairwise_prod(statd, x[None], y[None])[0])
+    # tests/pplacer/test_linear.ml:4-12 closed form: log 3 + log 4
+    m1 = np.array([[-1.0, 0.0, 1.0], [1000.0, 1.0, 3.0], [0.0, 0.0, 2.0]])
+    m2 = np.array([[-1.0, 1.0, 2.0], [1.0, 3.0, 0.0], [0.0, 1.0, 2.0]])
+    assert abs(linear.mat_masked_logdot(m1, m2, np.array([1, 0, 1], np.uint16))
+               - (math.log(3) + math.log(4))) < 1e-12
     # tests/pplacer/test_linear.ml:14-28 closed form through the GPU twin
     t1 = np.array([[[-1.0, 0.0, 1.0], [1000.0, 1.0, 3.0], [0.0, 0.0, 2.0]]] * 2)
     t2 = np.array([[[-1.0, 1.0, 0.0], [1.0, 3.0, 0.0], [0.0, 1.0, 2.0]],
@@ -134,11 +151,18 @@ def test_three_tax_optimize_matches_oracle(gtr50):
         d0 = [bl[l] / 2 for l in locs]
         ll, pe, di, calls, st = pl.optimize(queries, qj, locs, p0, d0, tol, 2.0, PP_FP64)
         assert np.all(st == 0)
+        n_exact = 0
         for k, (q, loc) in enumerate(jobs):
             ost, oll, ope, odi, ocalls = core.optimize(queries[q], loc, 0.1, bl[loc] / 2, tol)
             assert ost == 0
-            assert abs(ll[k] - oll) <= 1e-9 * abs(oll) + 1e-9 * max(1.0, 100 * tol), (k, ll[k], oll)
+            assert abs(ll[k] - oll) <= 1e-9 * abs(oll), (k, ll[k], oll)
             assert abs(pe[k] - ope) <= 2 * tol and abs(di[k] - odi) <= 2 * tol
+            # Three_tax.n_like_calls: the same count of log_like3 calls, the
+            # GSL set's three re-evaluations included, i.e. the same Brent path
+            assert calls[k] == ocalls, (k, calls[k], ocalls)
+            n_exact += (ll[k] == oll and pe[k] == ope and di[k] == odi)
+        print("tol %g: %d of %d jobs bit-equal to the oracle" % (tol, n_exact, len(jobs)))
+        assert n_exact >= 0.9 * len(jobs)
         ll32, pe32, di32, _, st32 = pl.optimize(queries, qj, locs, p0, d0, tol, 2.0, PP_FP32)
         assert np.all(st32 == 0)
         # FP32 probes change Brent's path: most jobs agree to FP32 rounding,
@@ -151,22 +175,33 @@ def test_three_tax_optimize_matches_oracle(gtr50):
 
 # ---- full placement --------------------------------------------------------------
 
-def _compare_placements(core, queries, got, status, ll_atol, fp64):
+def _compare_placements(core, queries, got, status, ll_atol, fp64, ties=None):
+    """Number of queries whose best placement is the oracle's edge, with the
+    likelihood within ll_atol (+1e-9 relative in FP64) and branch lengths
+    within twice final_tolerance (core.ml:14) in FP64.  A different best edge
+    is accepted only as a tie in the reference's own output: the oracle keeps
+    that edge too, with a likelihood within 1e-6 of its best (which of two
+    such edges comes first is decided by rounding); such queries are counted
+    in ties[0] when a list is given, and fail the test otherwise."""
     n_same_best = 0
+    bl_tol = 2e-5 if fp64 else 2e-4
     for q, query in enumerate(queries):
         st, want, _ = core.place(query)
         assert st == 0 and status[q] == 0
         g = got[q]
         assert len(g) > 0
-        if g[0].edge_num == want[0][0]:
-            n_same_best += 1
-        else:
-            # a different best edge is only acceptable as a near-tie
-            assert abs(g[0].likelihood - want[0][2]) < 1e-6, (q, g[0], want[0])
+        if g[0].edge_num != want[0][0]:
+            wmap = {p[0]: p for p in want}
+            tie = (ties is not None and g[0].edge_num in wmap
+                   and abs(wmap[g[0].edge_num][2] - want[0][2]) < 1e-6
+                   and abs(g[0].likelihood - want[0][2]) < 1e-6)
+            assert tie, (q, g[0], want[:2])
+            ties[0] += 1
             continue
+        n_same_best += 1
         assert abs(g[0].likelihood - want[0][2]) <= ll_atol + (1e-9 * abs(want[0][2]) if fp64 else 0)
-        assert abs(g[0].pendant_length - want[0][4]) < 2e-4
-        assert abs(g[0].distal_length - want[0][3]) < 2e-4
+        assert abs(g[0].pendant_length - want[0][4]) < bl_tol, (q, g[0], want[0])
+        assert abs(g[0].distal_length - want[0][3]) < bl_tol, (q, g[0], want[0])
         assert abs(sum(p.like_weight_ratio for p in g) - 1.0) < 1e-9
         wmap = {p[0]: p for p in want}
         for p in g:
@@ -190,8 +225,10 @@ def test_place_fp32_matches_oracle(gtr50):
     _print_case("place fp32")
     pl, core, queries = gtr50["pl"], gtr50["core"], gtr50["queries"]
     got, status, stats = pl.place(queries, precision=PP_FP32)
-    n = _compare_placements(core, queries, got, status, 1e-4, False)
-    assert n >= len(queries) - 1
+    ties = [0]
+    n = _compare_placements(core, queries, got, status, 1e-4, False, ties)
+    print("fp32: %d same best edge, %d ties of the reference's own output" % (n, ties[0]))
+    assert n + ties[0] == len(queries)
 
 
 def test_place_edge_cases(gtr50):
@@ -308,7 +345,8 @@ def test_full_size_placements(gtr2000):
     assert n == len(sub)
     got32, status32, stats32 = pl.place(sub, precision=PP_FP32)
     assert np.all(status32 == 0)
-    same = sum(g32[0].edge_num == g[0].edge_num for g32, g in zip(got32, got))
-    assert same >= len(sub) - 1
+    ties = [0]
+    n32 = _compare_placements(core, sub, got32, status32, 1e-4, False, ties)
+    assert n32 + ties[0] == len(sub)
     # the FP32 pitches do not inflate the work (no spurious non-finite probes)
     assert stats32.n_pitched <= 1.5 * stats.n_pitched

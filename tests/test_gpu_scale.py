@@ -53,4 +53,5 @@ def test_scale_placements(gtr10000):
     got, status, stats = pl.place(queries, precision=PP_FP64)
     assert _compare_placements(core, queries, got, status, 1e-9, True) == len(queries)
     got32, status32, _ = pl.place(queries, precision=PP_FP32)
-    assert _compare_placements(core, queries, got32, status32, 1e-4, False) >= len(queries) - 1
+    ties = [0]
+    assert _compare_placements(core, queries, got32, status32, 1e-4, False, ties) + ties[0] == len(queries)

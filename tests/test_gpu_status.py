@@ -10,10 +10,10 @@ way; the GPU must fail the same queries with the same status, and place the
 others.
 
 The FP32 form steers Brent with FP32 likelihoods, so the same drift can trip
-the assertion on other queries (119, 836, 870 and 1600 here).  The batch entry
-places such queries again in the FP64 form: the FP32 form then fails exactly
-the reference's 5, and its results for the 4 equal the FP64 form's bit for
-bit.
+the assertion on other queries (119, 836, 870 and 1600 here).  A pass places
+such queries again in the FP64 form, at the batch entry and at the resident
+entry alike: the FP32 form then fails exactly the reference's 5, and its
+results for the 4 equal the FP64 form's bit for bit.
 """
 import pytest
 
@@ -56,5 +56,13 @@ def test_prox_assertion_queries_fail_as_in_the_oracle(oracle):
         assert all(int(status32[i]) == 6 for i in FAILING)
         retried = [119, 836, 870, 1600]
         assert [_fields(got32[i]) for i in retried] == [_fields(got[i]) for i in retried]
+        _print_case("gtr50 seed 21, all 2000 queries, fp32, resident entry")
+        prefs = placer.default_prefs()
+        pl.upload_queries(queries)
+        stats = pl.place_resident(prefs, PP_FP32)
+        got_r, status_r = pl.download_results(len(queries), prefs.keep_at_most)
+        assert [i for i in range(len(queries)) if status_r[i] != 0] == FAILING
+        assert [_fields(g) for g in got_r] == [_fields(g) for g in got32]
+        assert stats.n_pitched > 0
     finally:
         pl.close()

@@ -1,20 +1,15 @@
 """Parity on a wide sample: 2,000 queries at 50 taxa and 300 at 2,000 taxa,
 every one compared with the oracle, in both forms.
 
-The strict bar (1e-9 relative in FP64) is what the other GPU tests hold on
-their cases.  Over a wide sample a few Brent runs part from the oracle's after
-a last-bit difference in one probe: the GPU sums a probe's per-site logs in
-wave order, the stub sequentially, and at a flat optimum one ulp can flip a
-comparison.  Such a run still ends within the Brent tolerance.  This test
-states how often that happens, from a measured sweep:
-
 * every status and every best edge equals the oracle's (at 50 taxa the oracle
-  fails 5 queries on the prox_bl assertion, and so does the GPU);
-* FP64: at least 99.5% of placements within 1e-9 relative (measured: 1995 of
-  2000 at 50 taxa, all 300 at 2,000 taxa), all within 1e-8 relative, branch
-  lengths within 2e-4;
-* FP32: all within 2e-4 absolute (measured maximum 1.02e-4), at least 99.9%
-  within 1e-4, branch lengths within 2e-4.
+  fails 5 queries on the prox_bl assertion, and so does the GPU, in both
+  forms: the FP32 form places its own assertion failures again in FP64);
+* FP64: every placement within 1e-9 relative of the oracle and branch lengths
+  within twice final_tolerance (core.ml:14).  The kernels sum each probe's
+  per-site logs in site order as the stub does, so probe values are the
+  oracle's to the last bit unless a per-site log differs by an ulp (the device
+  log against the host libm), and Brent takes the oracle's path;
+* FP32: every placement within 1e-4 absolute, branch lengths within 2e-4.
 """
 import pytest
 
@@ -53,6 +48,9 @@ def _sweep(oracle, name, seed, nq):
                 rel.append(ab[-1] / abs(w[0][2]))
                 bl.append(max(abs(g.pendant_length - w[0][4]), abs(g.distal_length - w[0][3])))
             out[form] = (rel, ab, bl)
+            print("sweep %s %s: %d placed, %d bit-equal likelihoods, max rel %.3g, max abs %.3g, "
+                  "max branch-length diff %.3g" % (name, form, len(rel), sum(r == 0 for r in rel),
+                                                    max(rel), max(ab), max(bl)), flush=True)
         return out
     finally:
         pl.close()
@@ -62,8 +60,6 @@ def _sweep(oracle, name, seed, nq):
 def test_wide_sample_against_the_oracle(oracle, name, seed, nq):
     out = _sweep(oracle, name, seed, nq)
     rel, ab, bl = out["fp64"]
-    assert sum(r <= 1e-9 for r in rel) >= 0.995 * len(rel)
-    assert max(rel) <= 1e-8 and max(bl) <= 2e-4
+    assert max(rel) <= 1e-9 and max(bl) <= 2e-5
     rel, ab, bl = out["fp32"]
-    assert max(ab) <= 2e-4 and max(bl) <= 2e-4
-    assert sum(a <= 1e-4 for a in ab) >= 0.999 * len(ab)
+    assert max(ab) <= 1e-4 and max(bl) <= 2e-4

@@ -234,7 +234,69 @@ def test_rank_is_stable_descending_with_nan_last(oracle):
     assert list(oracle.Core.rank(s)) == [1, 3, 5, 0, 6, 4, 2]
 
 
-# ---- Brent (GSL restatement): parity unpinned, closed-form checks ------------
+# ---- Brent (GSL restatement) --------------------------------------------------
+
+# The example of the GSL reference manual's "One Dimensional Minimization"
+# chapter (GSL 2.x, doc/examples/min.c): Brent on f(x) = cos(x) + 1 from
+# x = 2 in [0, 6], iterated until gsl_min_test_interval(a, b, 0.001, 0) holds.
+# Each row is the manual's printed line: iteration, [lower, upper], minimum,
+# minimum - pi, upper - lower, as "%5d [%.7f, %.7f] %.7f %+.7f %.7f".
+GSL_MANUAL_BRENT_ROWS = [
+    "    0 [0.0000000, 6.0000000] 2.0000000 -1.1415927 6.0000000",
+    "    1 [2.0000000, 6.0000000] 3.5278640 +0.3862713 4.0000000",
+    "    2 [2.0000000, 3.5278640] 3.1748217 +0.0332290 1.5278640",
+    "    3 [2.0000000, 3.1748217] 3.1264576 -0.0151351 1.1748217",
+    "    4 [3.1264576, 3.1748217] 3.1414743 -0.0001183 0.0483641",
+    "    5 [3.1414743, 3.1748217] 3.1415930 +0.0000004 0.0333474",
+    "    6 [3.1414743, 3.1415930] 3.1415927 +0.0000000 0.0001187",
+]
+
+
+def test_brent_iterates_reproduce_the_gsl_manual_example(oracle):
+    """Pins the restated GSL Brent step (golden section, parabolic step and its
+    admissibility test, the v/w/x history update) to GSL's own published
+    iterates: a wrong parabola, a wrong acceptance test or a wrong history
+    update moves a printed digit."""
+    b = oracle.GslBrent(lambda x: math.cos(x) + 1.0)
+    assert b.set(2.0, 0.0, 6.0) == 0
+    rows = []
+    converged_at = None
+    for it in range(0, 100):
+        if it > 0:
+            assert b.iterate() == 0
+        s = b.s
+        m, lo, hi = s.x_minimum, s.x_lower, s.x_upper
+        rows.append("%5d [%.7f, %.7f] %.7f %+.7f %.7f" % (it, lo, hi, m, m - math.pi, hi - lo))
+        if it > 0 and abs(hi - lo) < 0.001:  # gsl_min_test_interval(a, b, 0.001, 0.0)
+            converged_at = it
+            break
+    assert rows == GSL_MANUAL_BRENT_ROWS
+    assert converged_at == 6
+    # set evaluates f(lower), f(upper), f(minimum), then init evaluates f(v)
+    assert b.points[:4] == [0.0, 6.0, 2.0, 0.0 + 0.3819660 * 6.0]
+    # one evaluation per iteration
+    assert len(b.points) == 4 + 6
+
+
+def test_brent_set_errors_and_max_iter(oracle):
+    b = oracle.GslBrent(lambda x: math.cos(x) + 1.0)
+    assert b.set(2.0, 6.0, 0.0) == 2        # lower > upper: GSL_EINVAL
+    assert b.set(0.0, 0.0, 6.0) == 2        # minimum on the boundary
+    assert b.set(0.2, 0.0, 6.0) == 2        # f(0.2) > f(6): endpoints do not enclose a minimum
+    b = oracle.GslBrent(lambda x: float("nan") if x > 5 else math.cos(x) + 1.0)
+    assert b.set(2.0, 0.0, 6.0) == 1        # non-finite f(upper): GSL_EBADFUNC
+    # Minimization.brent's ExceededMaxIter (minimization.ml:95-99): the cos
+    # example needs 6 iterations at tolerance 1e-3, so 5 are too few; the
+    # start finder accepts x = 2 at once (f(2) < min(f(0), f(6)))
+    st, x, calls = oracle.brent(lambda v: math.cos(v) + 1.0, 2.0, 0.0, 6.0, 1e-3, max_iters=5)
+    assert st == 3
+    st, x, calls = oracle.brent(lambda v: math.cos(v) + 1.0, 2.0, 0.0, 6.0, 1e-3, max_iters=6)
+    assert st == 0 and abs(x - math.pi) < 1e-6
+    # f(right), f(left), f(start), then the GSL set's three and init's one, then 6 steps
+    assert calls[:7] == [6.0, 0.0, 2.0, 0.0, 6.0, 2.0, 0.3819660 * 6.0] and len(calls) == 13
+
+
+# ---- Brent: closed-form checks of the minimization.ml wrapper -----------------
 
 def test_brent_closed_forms(oracle):
     st, x, calls = oracle.brent(lambda v: (v - 0.3) ** 2, 0.9, 0.0, 2.0, 1e-6)
