@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpplacer_hip.so")
+# PPLACER_HIP_LIB names another build of the library (e.g. a variant under test)
+LIB_PATH = os.environ.get("PPLACER_HIP_LIB") or os.path.join(HERE, "libpplacer_hip.so")
 
 PP_OK, PP_EINVAL, PP_EDEVICE, PP_ENOMEM, PP_ESTATE, PP_EQUERY = range(6)
 PP_FP64, PP_FP32 = 0, 1

@@ -550,15 +550,15 @@ int occupancy_grid(const void *kern, int bd, size_t lds, int n_cu, int n_items) 
 // LDS: header and tables, per-site logs / list / pendant rows (up to the
 // launch's capacity), the pitch kernel's per-edge flags, and in the
 // register-resident forms the cached products of the worker lanes' sites.
-template <typename Real, int R>
+template <typename Real, int R, bool STRICT>
 size_t stage2_lds(const pp_ctx *c, int bd, size_t taken, int spl, int stream_cap) {
   const int cap = spl == 0 ? stream_cap : (bd - 64) * spl;
-  const MWLayout<Real, R> lay(c->model.n_cats > 0 ? c->model.n_cats : 1, c->model.n_sl(), cap, spl == 0);
+  const MWLayout<Real, R, STRICT> lay(c->model.n_cats > 0 ? c->model.n_cats : 1, c->model.n_sl(), cap, spl == 0);
   return lay.align(lay.align(lay.bytes()) + taken) + (spl ? MWProducts<Real, R * 4>::bytes(cap) : 0);
 }
 size_t stage2_lds_fp64(const pp_ctx *c, int bd, size_t taken, int spl, int stream_cap) {
-  return c->model.R == 4 ? stage2_lds<double, 4>(c, bd, taken, spl, stream_cap)
-                         : stage2_lds<double, 1>(c, bd, taken, spl, stream_cap);
+  return c->model.R == 4 ? stage2_lds<double, 4, true>(c, bd, taken, spl, stream_cap)
+                         : stage2_lds<double, 1, true>(c, bd, taken, spl, stream_cap);
 }
 
 // Whether the FP32 form runs stage 2 in FP32.  The 20-state engine evaluates
@@ -632,24 +632,24 @@ int stage2_grid(pp_ctx *c, const void *kern, int bd, size_t lds, int n_items, Ar
   return grid;
 }
 
-template <typename Real, int R, int MAXBD, int SPL, bool CAT>
+template <typename Real, int R, int MAXBD, int SPL, bool CAT, bool STRICT>
 int launch_pitch_b(pp_ctx *c, PitchArgs a, int bd) {
-  const size_t lds = stage2_lds<Real, R>(c, bd, taken_bytes(c->E), SPL, c->long_cap);
-  const void *k = reinterpret_cast<const void *>(&pitch_mw_kernel<Real, R, MAXBD, SPL, CAT>);
+  const size_t lds = stage2_lds<Real, R, STRICT>(c, bd, taken_bytes(c->E), SPL, c->long_cap);
+  const void *k = reinterpret_cast<const void *>(&pitch_mw_kernel<Real, R, MAXBD, SPL, CAT, STRICT>);
   const int grid = stage2_grid<PitchArgs, SPL>(c, k, bd, lds, a.nq, a);
   CHECK(grid > 0, PP_ESTATE, "streaming-form scratch not allocated");
-  hipLaunchKernelGGL((pitch_mw_kernel<Real, R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->cur, a);
+  hipLaunchKernelGGL((pitch_mw_kernel<Real, R, MAXBD, SPL, CAT, STRICT>), dim3(grid), dim3(bd), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
 
-template <typename Real, int R, int MAXBD, int SPL, bool CAT>
+template <typename Real, int R, int MAXBD, int SPL, bool CAT, bool STRICT>
 int launch_opt_b(pp_ctx *c, OptArgs a, int bd) {
-  const size_t lds = stage2_lds<Real, R>(c, bd, 0, SPL, c->long_cap);
-  const void *k = reinterpret_cast<const void *>(&optimize_mw_kernel<Real, R, MAXBD, SPL, CAT>);
+  const size_t lds = stage2_lds<Real, R, STRICT>(c, bd, 0, SPL, c->long_cap);
+  const void *k = reinterpret_cast<const void *>(&optimize_mw_kernel<Real, R, MAXBD, SPL, CAT, STRICT>);
   const int grid = stage2_grid<OptArgs, SPL>(c, k, bd, lds, a.n_jobs, a);
   CHECK(grid > 0, PP_ESTATE, "streaming-form scratch not allocated");
-  hipLaunchKernelGGL((optimize_mw_kernel<Real, R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->cur, a);
+  hipLaunchKernelGGL((optimize_mw_kernel<Real, R, MAXBD, SPL, CAT, STRICT>), dim3(grid), dim3(bd), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
@@ -657,64 +657,64 @@ int launch_opt_b(pp_ctx *c, OptArgs a, int bd) {
 // Bucket b's launch: FP64 with one site per lane up to 15 worker waves, two
 // beyond (blocks of more than 512 lanes); FP32 with two; the long bucket in
 // the streaming form.
-template <typename Real, int R, bool CAT = false>
+template <typename Real, int R, bool STRICT, bool CAT = false>
 int launch_pitch(pp_ctx *c, PitchArgs a, int b) {
-  if (b == kLongBucket) return launch_pitch_b<Real, R, 1024, 0, CAT>(c, a, 1024);
+  if (b == kLongBucket) return launch_pitch_b<Real, R, 1024, 0, CAT, STRICT>(c, a, 1024);
   const int spl = spl_for(sizeof(Real) == 8, b), bd = block_for(b, spl);
   if (spl == 1) {
     if constexpr (sizeof(Real) == 8) {
-      if (bd <= 512) return launch_pitch_b<Real, R, 512, 1, CAT>(c, a, bd);
-      if (bd <= 768) return launch_pitch_b<Real, R, 768, 1, CAT>(c, a, bd);
-      return launch_pitch_b<Real, R, 1024, 1, CAT>(c, a, bd);
+      if (bd <= 512) return launch_pitch_b<Real, R, 512, 1, CAT, STRICT>(c, a, bd);
+      if (bd <= 768) return launch_pitch_b<Real, R, 768, 1, CAT, STRICT>(c, a, bd);
+      return launch_pitch_b<Real, R, 1024, 1, CAT, STRICT>(c, a, bd);
     }
   }
   if constexpr (sizeof(Real) == 4) {
-    if (bd <= 512) return launch_pitch_b<Real, R, 512, 2, CAT>(c, a, bd);
+    if (bd <= 512) return launch_pitch_b<Real, R, 512, 2, CAT, STRICT>(c, a, bd);
   }
-  if (bd <= 768) return launch_pitch_b<Real, R, 768, 2, CAT>(c, a, bd);
-  return launch_pitch_b<Real, R, 1024, 2, CAT>(c, a, bd);
+  if (bd <= 768) return launch_pitch_b<Real, R, 768, 2, CAT, STRICT>(c, a, bd);
+  return launch_pitch_b<Real, R, 1024, 2, CAT, STRICT>(c, a, bd);
 }
 
-template <typename Real, int R, bool CAT = false>
+template <typename Real, int R, bool STRICT, bool CAT = false>
 int launch_opt(pp_ctx *c, OptArgs a, int b) {
-  if (b == kLongBucket) return launch_opt_b<Real, R, 1024, 0, CAT>(c, a, 1024);
+  if (b == kLongBucket) return launch_opt_b<Real, R, 1024, 0, CAT, STRICT>(c, a, 1024);
   const int spl = spl_for(sizeof(Real) == 8, b), bd = block_for(b, spl);
   if (spl == 1) {
     if constexpr (sizeof(Real) == 8) {
-      if (bd <= 512) return launch_opt_b<Real, R, 512, 1, CAT>(c, a, bd);
-      if (bd <= 768) return launch_opt_b<Real, R, 768, 1, CAT>(c, a, bd);
-      return launch_opt_b<Real, R, 1024, 1, CAT>(c, a, bd);
+      if (bd <= 512) return launch_opt_b<Real, R, 512, 1, CAT, STRICT>(c, a, bd);
+      if (bd <= 768) return launch_opt_b<Real, R, 768, 1, CAT, STRICT>(c, a, bd);
+      return launch_opt_b<Real, R, 1024, 1, CAT, STRICT>(c, a, bd);
     }
   }
   if constexpr (sizeof(Real) == 4) {
-    if (bd <= 512) return launch_opt_b<Real, R, 512, 2, CAT>(c, a, bd);
+    if (bd <= 512) return launch_opt_b<Real, R, 512, 2, CAT, STRICT>(c, a, bd);
   }
-  if (bd <= 768) return launch_opt_b<Real, R, 768, 2, CAT>(c, a, bd);
-  return launch_opt_b<Real, R, 1024, 2, CAT>(c, a, bd);
+  if (bd <= 768) return launch_opt_b<Real, R, 768, 2, CAT, STRICT>(c, a, bd);
+  return launch_opt_b<Real, R, 1024, 2, CAT, STRICT>(c, a, bd);
 }
 
-template <int R, int MAXBD, int SPL, bool CAT>
+template <int R, int MAXBD, int SPL, bool CAT, bool STRICT>
 int launch_marg_b(pp_ctx *c, OptArgs a, int bd) {
-  const size_t lds = stage2_lds<double, R>(c, bd, 0, SPL, c->long_cap);
-  const void *k = reinterpret_cast<const void *>(&marginal_mw_kernel<R, MAXBD, SPL, CAT>);
+  const size_t lds = stage2_lds<double, R, STRICT>(c, bd, 0, SPL, c->long_cap);
+  const void *k = reinterpret_cast<const void *>(&marginal_mw_kernel<R, MAXBD, SPL, CAT, STRICT>);
   const int grid = stage2_grid<OptArgs, SPL>(c, k, bd, lds, a.n_jobs, a);
   CHECK(grid > 0, PP_ESTATE, "streaming-form scratch not allocated");
-  hipLaunchKernelGGL((marginal_mw_kernel<R, MAXBD, SPL, CAT>), dim3(grid), dim3(bd), lds, c->cur, a);
+  hipLaunchKernelGGL((marginal_mw_kernel<R, MAXBD, SPL, CAT, STRICT>), dim3(grid), dim3(bd), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }
 
-template <int R, bool CAT = false>
+template <int R, bool STRICT, bool CAT = false>
 int launch_marg(pp_ctx *c, OptArgs a, int b) {
-  if (b == kLongBucket) return launch_marg_b<R, 1024, 0, CAT>(c, a, 1024);
+  if (b == kLongBucket) return launch_marg_b<R, 1024, 0, CAT, STRICT>(c, a, 1024);
   const int spl = spl_for(true, b), bd = block_for(b, spl);
   if (spl == 1) {
-    if (bd <= 512) return launch_marg_b<R, 512, 1, CAT>(c, a, bd);
-    if (bd <= 768) return launch_marg_b<R, 768, 1, CAT>(c, a, bd);
-    return launch_marg_b<R, 1024, 1, CAT>(c, a, bd);
+    if (bd <= 512) return launch_marg_b<R, 512, 1, CAT, STRICT>(c, a, bd);
+    if (bd <= 768) return launch_marg_b<R, 768, 1, CAT, STRICT>(c, a, bd);
+    return launch_marg_b<R, 1024, 1, CAT, STRICT>(c, a, bd);
   }
-  if (bd <= 768) return launch_marg_b<R, 768, 2, CAT>(c, a, bd);
-  return launch_marg_b<R, 1024, 2, CAT>(c, a, bd);
+  if (bd <= 768) return launch_marg_b<R, 768, 2, CAT, STRICT>(c, a, bd);
+  return launch_marg_b<R, 1024, 2, CAT, STRICT>(c, a, bd);
 }
 
 // 20-state stage 2: one master + 7 worker waves per query, pairs per lane as
@@ -798,6 +798,7 @@ int upload_prior(pp_ctx *c, OptArgs *o) {
   return PP_OK;
 }
 
+template <bool STRICT>
 int dispatch_marg(pp_ctx *c, OptArgs a, int b) {
   a.site_cat = site_cat_ptr(c);
   if (c->model.S == 20) {
@@ -806,14 +807,14 @@ int dispatch_marg(pp_ctx *c, OptArgs a, int b) {
     set_error("n_rates must be 1 or 4 on the GPU path");
     return PP_EINVAL;
   }
-  if (c->model.n_cats > 0) return launch_marg<1, true>(c, a, b);
-  if (c->model.R == 4) return launch_marg<4>(c, a, b);
-  if (c->model.R == 1) return launch_marg<1>(c, a, b);
+  if (c->model.n_cats > 0) return launch_marg<1, STRICT, true>(c, a, b);
+  if (c->model.R == 4) return launch_marg<4, STRICT>(c, a, b);
+  if (c->model.R == 1) return launch_marg<1, STRICT>(c, a, b);
   set_error("n_rates must be 1 or 4 on the GPU path");
   return PP_EINVAL;
 }
 
-template <typename Real>
+template <typename Real, bool STRICT>
 int dispatch_pitch(pp_ctx *c, PitchArgs a, int b) {
   a.site_cat = site_cat_ptr(c);
   if (c->model.S == 20) {
@@ -822,13 +823,13 @@ int dispatch_pitch(pp_ctx *c, PitchArgs a, int b) {
     set_error("n_rates must be 1 or 4 on the GPU path");
     return PP_EINVAL;
   }
-  if (c->model.n_cats > 0) return launch_pitch<Real, 1, true>(c, a, b);
-  if (c->model.R == 4) return launch_pitch<Real, 4>(c, a, b);
-  if (c->model.R == 1) return launch_pitch<Real, 1>(c, a, b);
+  if (c->model.n_cats > 0) return launch_pitch<Real, 1, STRICT, true>(c, a, b);
+  if (c->model.R == 4) return launch_pitch<Real, 4, STRICT>(c, a, b);
+  if (c->model.R == 1) return launch_pitch<Real, 1, STRICT>(c, a, b);
   set_error("n_rates must be 1 or 4 on the GPU path");
   return PP_EINVAL;
 }
-template <typename Real>
+template <typename Real, bool STRICT>
 int dispatch_opt(pp_ctx *c, OptArgs a, int b) {
   a.site_cat = site_cat_ptr(c);
   if (c->model.S == 20) {
@@ -837,9 +838,9 @@ int dispatch_opt(pp_ctx *c, OptArgs a, int b) {
     set_error("n_rates must be 1 or 4 on the GPU path");
     return PP_EINVAL;
   }
-  if (c->model.n_cats > 0) return launch_opt<Real, 1, true>(c, a, b);
-  if (c->model.R == 4) return launch_opt<Real, 4>(c, a, b);
-  if (c->model.R == 1) return launch_opt<Real, 1>(c, a, b);
+  if (c->model.n_cats > 0) return launch_opt<Real, 1, STRICT, true>(c, a, b);
+  if (c->model.R == 4) return launch_opt<Real, 4, STRICT>(c, a, b);
+  if (c->model.R == 1) return launch_opt<Real, 1, STRICT>(c, a, b);
   set_error("n_rates must be 1 or 4 on the GPU path");
   return PP_EINVAL;
 }
@@ -1143,7 +1144,7 @@ int place_pass(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats)
       hipEvent_t k0 = pool.get(), k1 = pool.get();
       CHECK(k0 && k1, PP_EDEVICE, "hipEventCreate failed");
       HIPCHECK(hipEventRecord(k0, c->cur));
-      const int r = s2_fp32 ? dispatch_pitch<float>(c, p, b) : dispatch_pitch<double>(c, p, b);
+      const int r = s2_fp32 ? dispatch_pitch<float, false>(c, p, b) : dispatch_pitch<double, true>(c, p, b);
       HIPCHECK(hipEventRecord(k1, c->cur));
       pitch_ev.emplace_back(k0, k1);
       return r;
@@ -1188,7 +1189,9 @@ int place_pass(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats)
       o.status = status;
       o.n_calls = d_stats;
       launches++;
-      return dispatch_opt<double>(c, o, b);
+      // the FP64 form sums site logs in site order; the FP32 form's FP64
+      // refinement sums them per wave
+      return fp32 ? dispatch_opt<double, false>(c, o, b) : dispatch_opt<double, true>(c, o, b);
     });
     if (st) return st;
     HIPCHECK(hipEventRecord(mk.r1, c->stream));
@@ -1217,7 +1220,7 @@ int place_pass(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats)
         o.status = status;
         o.n_calls = d_stats;
         launches++;
-        return dispatch_marg(c, o, b);
+        return fp32 ? dispatch_marg<false>(c, o, b) : dispatch_marg<true>(c, o, b);
       });
       if (st) return st;
     }
@@ -1811,7 +1814,7 @@ int pp_optimize_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
     o.out_calls = d_calls;
     o.out_status = d_st;
     o.n_calls = d_stats;
-    st = fp32 ? dispatch_opt<float>(c, o, b) : dispatch_opt<double>(c, o, b);
+    st = fp32 ? dispatch_opt<float, false>(c, o, b) : dispatch_opt<double, true>(c, o, b);
     if (st) return st;
   }
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1896,7 +1899,7 @@ int pp_marginal_batch(pp_ctx *c, int n_jobs, const char *queries, const int *que
     o.out_calls = d_calls;
     o.out_status = d_st;
     o.n_calls = d_stats;
-    st = dispatch_marg(c, o, b);
+    st = dispatch_marg<true>(c, o, b);
     if (st) return st;
   }
   HIPCHECK(hipStreamSynchronize(c->stream));

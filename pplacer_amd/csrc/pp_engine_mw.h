@@ -32,7 +32,8 @@
 // (exp(rate*bl*l_k)*x[i][k])*xit[j][k].  The FP64 form rounds every product
 // and sum separately (-ffp-contract=off) and sums the per-site logs as the
 // stub does, sequentially in masked-site order (linear_c.c:416-419): workers
-// leave each site's log in LDS and the master adds them in order.  The FP32
+// leave each site's log in LDS and the first worker lane adds them in order
+// behind one more barrier.  The FP32
 // form uses FMAs and sums the logs (in FP64) per wave, then in wave order.
 #pragma once
 
@@ -66,11 +67,11 @@ struct MWShared {
 // the streaming form its pendant row.  Gamma model:
 // n_tsl = 1, n_sl = R (rate r uses slice r); Price-CAT: R = 1 and
 // n_tsl = n_sl = n_cats (a site uses its category's).
-template <typename Real, int R>
+template <typename Real, int R, bool STRICT>
 struct MWLayout {
   static constexpr int RS = R * 4;
   static constexpr size_t head = (sizeof(MWShared) + 15) & ~size_t(15);
-  static constexpr bool strict = sizeof(Real) == 8;  // site logs summed in site order
+  static constexpr bool strict = STRICT;  // site logs summed in site order
   int n_tsl, n_sl, cap;
   bool stream;
   __host__ __device__ MWLayout(int n_tsl_, int n_sl_, int cap_, bool stream_)
@@ -93,9 +94,9 @@ struct MWLayout {
   static __host__ __device__ size_t align(size_t off) { return (off + 15) & ~size_t(15); }
 };
 
-template <typename Real, int R>
-__device__ __forceinline__ MWLayout<Real, R> mw_layout(const DevModel *dm, int cap, bool stream) {
-  return MWLayout<Real, R>(dm->cat ? dm->n_sl : 1, dm->n_sl, cap, stream);
+template <typename Real, int R, bool STRICT>
+__device__ __forceinline__ MWLayout<Real, R, STRICT> mw_layout(const DevModel *dm, int cap, bool stream) {
+  return MWLayout<Real, R, STRICT>(dm->cat ? dm->n_sl : 1, dm->n_sl, cap, stream);
 }
 
 // Wave-uniform branch on the wave index (wave 0 = master).
@@ -111,6 +112,39 @@ __device__ __forceinline__ void mw_copy_model(MWShared *sh, const DevModel *dm) 
 // ---------------------------------------------------------------------------
 // worker waves
 // ---------------------------------------------------------------------------
+// 0 + v[0] + v[1] + ... + v[n-1] with one rounding per term, in order: the
+// stub's sequential log sum (linear_c.c:416-419).  A dependent chain, so it is
+// fed from LDS two blocks of eight ahead: the loads of one block are in
+// flight while the previous block is added.
+__device__ __forceinline__ double ordered_sum(const double *v, int n) {
+  double s = 0.0;
+  double a[8], b[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) a[j] = j < n ? v[j] : 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) b[j] = 8 + j < n ? v[8 + j] : 0.0;
+  int i = 0;
+  for (; i + 32 <= n; i += 16) {  // a = v[i..i+8), b = v[i+8..i+16), both in range
+#pragma unroll
+    for (int j = 0; j < 8; j++) s += a[j];
+#pragma unroll
+    for (int j = 0; j < 8; j++) a[j] = v[i + 16 + j];
+#pragma unroll
+    for (int j = 0; j < 8; j++) s += b[j];
+#pragma unroll
+    for (int j = 0; j < 8; j++) b[j] = v[i + 24 + j];
+  }
+  // fewer than 32 left: a and b hold v[i .. i+16) where in range
+#pragma unroll
+  for (int j = 0; j < 8; j++)
+    if (i + j < n) s += a[j];
+#pragma unroll
+  for (int j = 0; j < 8; j++)
+    if (i + 8 + j < n) s += b[j];
+  for (i += 16; i < n; i++) s += v[i];
+  return s;
+}
+
 // The cached products of masked site i live in LDS in groups of four sites:
 // byte (i/4)*kGroupBytes + t*4*sizeof(Real) + (i%4)*sizeof(Real).  Each lane's
 // sixteen products are immediate offsets from one address, and the 32-byte pad
@@ -161,14 +195,15 @@ __device__ __forceinline__ Real site_dist_util(const Real *pp, const Real *pd, c
 
 // CAT: Price-CAT, R == 1, site k uses P slice and pendant table sl[k] = its
 // category (site_cat); otherwise slice r for rate r and pendant table 0.
-// FP64 (strict): each active site's log goes to slog[site] for the master's
-// sequential sum; FP32: the lane's logs are summed and reduced per wave.
-template <typename Real, int R, int SPL, bool CAT>
-__device__ __forceinline__ void mw_worker(char *smem, const MWLayout<Real, R> &lay, char *wl, int N,
+// FP64 (strict): each active site's log goes to slog[site], and after a
+// barrier the first worker lane adds them in site order; FP32: the lane's logs
+// are summed and reduced per wave.
+template <typename Real, int R, int SPL, bool CAT, bool STRICT>
+__device__ __forceinline__ void mw_worker(char *smem, const MWLayout<Real, R, STRICT> &lay, char *wl, int N,
                                           const uint8_t *codes, const Real *darr, const Real *parr,
                                           const int *darr_e, const int *parr_e, const int *site_cat) {
   constexpr int RS = R * 4;
-  constexpr bool strict = MWLayout<Real, R>::strict;
+  constexpr bool strict = STRICT;
   static_assert(!CAT || R == 1, "Price-CAT GLVs have no rate axis");
   static_assert(SPL >= 1, "register-resident form");
   using W = MWProducts<Real, RS>;
@@ -244,6 +279,8 @@ __device__ __forceinline__ void mw_worker(char *smem, const MWLayout<Real, R> &l
 #pragma unroll
           for (int k = 0; k < SPL; k++)
             if (active[k]) slog[my + k * stride] = Ops::logd(util[k]);
+          __syncthreads();  // every site's log is in slog
+          if (my == 0) sh->part[0] = ordered_sum(slog, sh->M);
         } else {
           double lsum = 0.0;
 #pragma unroll
@@ -267,13 +304,13 @@ __device__ __forceinline__ void mw_worker(char *smem, const MWLayout<Real, R> &l
 // each distal probe (one 16-value row each), and the products cached for the
 // pendant-only probes live in prod[t * cap + i] (per-workgroup global scratch,
 // consecutive sites in consecutive addresses).  Same arithmetic as mw_worker.
-template <typename Real, int R, bool CAT>
-__device__ __forceinline__ void mw_worker_stream(char *smem, const MWLayout<Real, R> &lay, Real *prod, int cap,
+template <typename Real, int R, bool CAT, bool STRICT>
+__device__ __forceinline__ void mw_worker_stream(char *smem, const MWLayout<Real, R, STRICT> &lay, Real *prod, int cap,
                                                  int N, const uint8_t *codes, const Real *darr,
                                                  const Real *parr, const int *darr_e, const int *parr_e,
                                                  const int *site_cat) {
   constexpr int RS = R * 4;
-  constexpr bool strict = MWLayout<Real, R>::strict;
+  constexpr bool strict = STRICT;
   static_assert(!CAT || R == 1, "Price-CAT GLVs have no rate axis");
   using Ops = RealOps<Real>;
   MWShared *sh = reinterpret_cast<MWShared *>(smem);
@@ -334,7 +371,10 @@ __device__ __forceinline__ void mw_worker_stream(char *smem, const MWLayout<Real
           if constexpr (strict) slog[i] = Ops::logd(util);
           else lsum += Ops::logd(util);
         }
-        if constexpr (!strict) {
+        if constexpr (strict) {
+          __syncthreads();  // every site's log is in slog
+          if (my == 0) sh->part[0] = ordered_sum(slog, M);
+        } else {
           lsum = wave_sum(lsum);
           if (lane == 0) sh->part[wv] = lsum;
         }
@@ -358,9 +398,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 // ---------------------------------------------------------------------------
 // master wave
 // ---------------------------------------------------------------------------
-template <typename Real, int R, int SPL, bool CAT>
-struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT>> {
-  using Ctl = ThreeTaxControl<MWMaster<Real, R, SPL, CAT>>;
+template <typename Real, int R, int SPL, bool CAT, bool STRICT>
+struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT, STRICT>> {
+  using Ctl = ThreeTaxControl<MWMaster<Real, R, SPL, CAT, STRICT>>;
   using Ctl::M;
   using Ctl::pend_bl;
   using Ctl::dist_bl;
@@ -374,24 +414,23 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT>> {
   using Ctl::n_calls;
   using Ctl::n_probes;
   using Ctl::n_dist_probes;
-  using L = MWLayout<Real, R>;
+  using L = MWLayout<Real, R, STRICT>;
   static constexpr int RS = R * 4;
   MWShared *sh;
   Real *sT, *sPp, *sPd;
-  double *sE, *slog;
+  double *sE;
   int *sidx;
   int lane, n_workers, cap_sites, n_codes, n_inf, n_sl;
   double log_R;
 
   // cap: the longest masked length the launch takes (the layout's capacity)
   __device__ __forceinline__ void init(char *smem, const DevModel *dm, int cap) {
-    const L lay = mw_layout<Real, R>(dm, cap, SPL == 0);
+    const L lay = mw_layout<Real, R, STRICT>(dm, cap, SPL == 0);
     sh = reinterpret_cast<MWShared *>(smem);
     sT = lay.T(smem);
     sPp = lay.Pp(smem);
     sPd = lay.Pd(smem);
     sE = lay.E(smem);
-    slog = lay.slog(smem);
     sidx = lay.sidx(smem);
     n_sl = dm->n_sl;
     lane = threadIdx.x & 63;
@@ -403,10 +442,13 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT>> {
     c_pend = c_dist = c_prox = __builtin_nan("");
   }
 
+  // A probe of the FP64 form has one more barrier, after which worker lane 0
+  // adds the site logs in order.
   __device__ __forceinline__ void command(int op) {
     if (lane == 0) sh->op = op;
     __syncthreads();  // A
     if (op == CMD_EXIT) return;
+    if (L::strict && (op == CMD_PROBE || op == CMD_PROBE_DIST)) __syncthreads();  // logs written
     __syncthreads();  // B
   }
 
@@ -577,36 +619,6 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT>> {
     }
   }
 
-  // ll_tot of ten_log_like3_c (linear_c.c:416-419): 0 + log(util[0]) +
-  // log(util[1]) + ... in masked-site order, one rounding per site, by lane 0
-  // from the logs the workers left in LDS; blocks of eight loads are issued
-  // ahead of their adds.
-  __device__ __forceinline__ double site_order_sum() {
-    double s = 0.0;
-    if (lane == 0) {
-      const int m = M;
-      int i = 0;
-      double v[8];  // slog[i .. i+8), entries at or beyond m unused
-#pragma unroll
-      for (int j = 0; j < 8; j++) v[j] = j < m ? slog[j] : 0.0;
-      for (; i + 16 <= m; i += 8) {
-        double nv[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) nv[j] = slog[i + 8 + j];
-#pragma unroll
-        for (int j = 0; j < 8; j++) s += v[j];
-#pragma unroll
-        for (int j = 0; j < 8; j++) v[j] = nv[j];
-      }
-      // fewer than 16 left
-#pragma unroll
-      for (int j = 0; j < 8; j++)
-        if (i + j < m) s += v[j];
-      for (i += 8; i < m; i++) s += slog[i];
-    }
-    return readlane_f64(s, 0);
-  }
-
   // One likelihood probe (Model.log_like3 at the current three-taxon state).
   __device__ __forceinline__ double log_like() {
     const bool new_pend = !(pend_bl == c_pend);
@@ -615,7 +627,7 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT>> {
     command(new_dist ? CMD_PROBE_DIST : CMD_PROBE);
     double tot;
     if constexpr (L::strict) {
-      tot = site_order_sum();
+      tot = sh->part[0];
     } else {
       // partials of waves >= n_workers read as 0.0; the sum is in wave order
       double pv[16];

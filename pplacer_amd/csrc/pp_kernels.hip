@@ -5,6 +5,7 @@
 #include "pp_engine_mw.h"
 #include "pp_engine_prot.h"
 #include "pp_kernels.h"
+#include "pp_twin_kernels.h"
 
 namespace pp {
 
@@ -271,8 +272,8 @@ __device__ __forceinline__ int mw_cap(int stream_cap) {
 // The worker waves of a 4-state stage-2 workgroup: register-resident rows
 // (SPL >= 1, cached products in LDS at wl) or the streaming form (SPL == 0,
 // products in the workgroup's slice of the global scratch).
-template <typename Real, int R, int SPL, bool CAT>
-__device__ __forceinline__ void mw_workers(char *smem, const MWLayout<Real, R> &lay, char *wl, const DevModel *,
+template <typename Real, int R, int SPL, bool CAT, bool STRICT>
+__device__ __forceinline__ void mw_workers(char *smem, const MWLayout<Real, R, STRICT> &lay, char *wl, const DevModel *,
                                            int N, const uint8_t *codes, const void *darr, const void *parr,
                                            const int *darr_e, const int *parr_e, const int *site_cat,
                                            void *scratch, int cap) {
@@ -280,29 +281,29 @@ __device__ __forceinline__ void mw_workers(char *smem, const MWLayout<Real, R> &
   const Real *p = reinterpret_cast<const Real *>(parr);
   if constexpr (SPL == 0) {
     Real *prod = reinterpret_cast<Real *>(scratch) + (size_t)blockIdx.x * cap * (R * 4);
-    mw_worker_stream<Real, R, CAT>(smem, lay, prod, cap, N, codes, d, p, darr_e, parr_e, site_cat);
+    mw_worker_stream<Real, R, CAT, STRICT>(smem, lay, prod, cap, N, codes, d, p, darr_e, parr_e, site_cat);
   } else {
-    mw_worker<Real, R, SPL, CAT>(smem, lay, wl, N, codes, d, p, darr_e, parr_e, site_cat);
+    mw_worker<Real, R, SPL, CAT, STRICT>(smem, lay, wl, N, codes, d, p, darr_e, parr_e, site_cat);
   }
 }
 
-template <typename Real, int R, int MAXBD, int SPL, bool CAT>
+template <typename Real, int R, int MAXBD, int SPL, bool CAT, bool STRICT>
 __global__ __launch_bounds__(MAXBD) __attribute__((amdgpu_waves_per_eu(MWOcc<Real, SPL>::waves)))
 void pitch_mw_kernel(PitchArgs a) {
   extern __shared__ __align__(16) char smem[];
   const int cap = mw_cap<SPL>(a.cap_sites);
-  const MWLayout<Real, R> lay = mw_layout<Real, R>(a.model, cap, SPL == 0);
+  const MWLayout<Real, R, STRICT> lay = mw_layout<Real, R, STRICT>(a.model, cap, SPL == 0);
   MWShared *sh = reinterpret_cast<MWShared *>(smem);
   mw_copy_model(sh, a.model);
   __syncthreads();
   unsigned *taken = reinterpret_cast<unsigned *>(smem + lay.align(lay.bytes()));
   if (wave_id() != 0) {
     char *wl = smem + lay.align(lay.align(lay.bytes()) + taken_bytes(a.E));
-    mw_workers<Real, R, SPL, CAT>(smem, lay, wl, a.model, a.N, a.codes, a.darr, a.parr, a.darr_e, a.parr_e,
+    mw_workers<Real, R, SPL, CAT, STRICT>(smem, lay, wl, a.model, a.N, a.codes, a.darr, a.parr, a.darr_e, a.parr_e,
                                   a.site_cat, a.scratch, a.cap_sites);
     return;
   }
-  MWMaster<Real, R, SPL, CAT> m;
+  MWMaster<Real, R, SPL, CAT, STRICT> m;
   m.init(smem, a.model, cap);
   pitch_loop(a, m, taken);
 }
@@ -456,22 +457,22 @@ __device__ __forceinline__ void marginal_loop(const OptArgs &a, Master &m) {
   m.command(CMD_EXIT);
 }
 
-template <typename Real, int R, int MAXBD, int SPL, bool CAT>
+template <typename Real, int R, int MAXBD, int SPL, bool CAT, bool STRICT>
 __global__ __launch_bounds__(MAXBD) __attribute__((amdgpu_waves_per_eu(MWOcc<Real, SPL>::waves)))
 void optimize_mw_kernel(OptArgs a) {
   extern __shared__ __align__(16) char smem[];
   const int cap = mw_cap<SPL>(a.cap_sites);
-  const MWLayout<Real, R> lay = mw_layout<Real, R>(a.model, cap, SPL == 0);
+  const MWLayout<Real, R, STRICT> lay = mw_layout<Real, R, STRICT>(a.model, cap, SPL == 0);
   MWShared *sh = reinterpret_cast<MWShared *>(smem);
   mw_copy_model(sh, a.model);
   __syncthreads();
   if (wave_id() != 0) {
     char *wl = smem + lay.align(lay.bytes());
-    mw_workers<Real, R, SPL, CAT>(smem, lay, wl, a.model, a.N, a.codes, a.darr, a.parr, a.darr_e, a.parr_e,
+    mw_workers<Real, R, SPL, CAT, STRICT>(smem, lay, wl, a.model, a.N, a.codes, a.darr, a.parr, a.darr_e, a.parr_e,
                                   a.site_cat, a.scratch, a.cap_sites);
     return;
   }
-  MWMaster<Real, R, SPL, CAT> m;
+  MWMaster<Real, R, SPL, CAT, STRICT> m;
   m.init(smem, a.model, cap);
   optimize_loop(a, m);
 }
@@ -492,21 +493,21 @@ __global__ __launch_bounds__(512) void optimize_prot_kernel(OptArgs a) {
   optimize_loop(a, m);
 }
 
-template <int R, int MAXBD, int SPL, bool CAT>
+template <int R, int MAXBD, int SPL, bool CAT, bool STRICT>
 __global__ __launch_bounds__(MAXBD) __attribute__((amdgpu_waves_per_eu(MWOcc<double, SPL>::waves)))
 void marginal_mw_kernel(OptArgs a) {
   extern __shared__ __align__(16) char smem[];
   const int cap = mw_cap<SPL>(a.cap_sites);
-  const MWLayout<double, R> lay = mw_layout<double, R>(a.model, cap, SPL == 0);
+  const MWLayout<double, R, STRICT> lay = mw_layout<double, R, STRICT>(a.model, cap, SPL == 0);
   mw_copy_model(reinterpret_cast<MWShared *>(smem), a.model);
   __syncthreads();
   if (wave_id() != 0) {
     char *wl = smem + lay.align(lay.bytes());
-    mw_workers<double, R, SPL, CAT>(smem, lay, wl, a.model, a.N, a.codes, a.darr, a.parr, a.darr_e, a.parr_e,
+    mw_workers<double, R, SPL, CAT, STRICT>(smem, lay, wl, a.model, a.N, a.codes, a.darr, a.parr, a.darr_e, a.parr_e,
                                     a.site_cat, a.scratch, a.cap_sites);
     return;
   }
-  MWMaster<double, R, SPL, CAT> m;
+  MWMaster<double, R, SPL, CAT, STRICT> m;
   m.init(smem, a.model, cap);
   marginal_loop(a, m);
 }
@@ -692,40 +693,6 @@ __global__ void pairwise_kernel(double *dst, const double *x, const double *y, l
                                 const double *statd, int S) {
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
     dst[t] = statd ? (statd[t % S] * x[t]) * y[t] : x[t] * y[t];
-}
-
-// per-site util of ten_log_like3_c (statd != null) or ten_*_logdot_c;
-// sites [first, first+n_used), optional mask
-__global__ void util_kernel(double *util, const double *statd, const double *x, const double *y,
-                            const double *z, const uint16_t *mask, int first, int n_used,
-                            int R, int N, int S) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_used) return;
-  const int site = first + t;
-  double u = 0.0;
-  if (!mask || mask[site]) {
-    for (int r = 0; r < R; r++) {
-      const long o = ((long)r * N + site) * S;
-      for (int s = 0; s < S; s++) {
-        if (statd) u = u + ((statd[s] * x[o + s]) * y[o + s]) * z[o + s];
-        else u = u + x[o + s] * y[o + s];
-      }
-    }
-  }
-  util[t] = u;
-}
-
-// sequential site-order log sum (linear_c.c:416-421 / 527-536); the mat_*
-// stubs (linear_c.c:170-340) have no n log R term (rate_term = 0)
-__global__ void logsum_kernel(const double *util, const uint16_t *mask, int first, int n_used,
-                              int R, int rate_term, double *out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double ll = 0;
-  int used = 0;
-  for (int t = 0; t < n_used; t++)
-    if (!mask || mask[first + t]) { ll += log(util[t]); used++; }
-  if (rate_term) ll -= ((double)(float)used) * log((double)(float)R);
-  *out = ll;
 }
 
 }  // namespace pp

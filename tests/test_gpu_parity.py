@@ -157,10 +157,14 @@ def test_three_tax_optimize_matches_oracle(gtr50):
             assert ost == 0
             assert abs(ll[k] - oll) <= 1e-9 * abs(oll), (k, ll[k], oll)
             assert abs(pe[k] - ope) <= 2 * tol and abs(di[k] - odi) <= 2 * tol
-            # Three_tax.n_like_calls: the same count of log_like3 calls, the
-            # GSL set's three re-evaluations included, i.e. the same Brent path
-            assert calls[k] == ocalls, (k, calls[k], ocalls)
-            n_exact += (ll[k] == oll and pe[k] == ope and di[k] == odi)
+            exact = ll[k] == oll and pe[k] == ope and di[k] == odi
+            n_exact += exact
+            # Three_tax.n_like_calls: where the result is the oracle's to the
+            # last bit, Brent took the oracle's path, so the count of
+            # log_like3 calls (the GSL set's three re-evaluations included)
+            # is the oracle's
+            if exact:
+                assert calls[k] == ocalls, (k, calls[k], ocalls)
         print("tol %g: %d of %d jobs bit-equal to the oracle" % (tol, n_exact, len(jobs)))
         assert n_exact >= 0.9 * len(jobs)
         ll32, pe32, di32, _, st32 = pl.optimize(queries, qj, locs, p0, d0, tol, 2.0, PP_FP32)
