@@ -95,6 +95,19 @@ int pp_ten_masked_logdot(const double *x, const double *y, const uint16_t *mask,
 int pp_ten_bounded_logdot(const double *x, const double *y, int first, int last, double *util,
                           int n_rates, int n_sites, int n_states, double *out);
 
+/* Allocation-free variant of pp_ten_masked_logdot.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * stream capture, serving loops).  x, y ([n_rates][n_sites][n_states]), mask
+ * ([n_sites]) and out (one double) are device pointers; the work is enqueued on
+ * `stream` (a hipStream_t, NULL for the default stream) and *out is written
+ * when the stream reaches it.  The per-site util vector lives in a static device
+ * array of 262,144 doubles per GPU (2 MiB, held until the process ends), so
+ * n_sites above that returns PP_EINVAL; calls that overlap on one GPU share that
+ * array and race. */
+int pp_ten_masked_logdot_noalloc(const double *x, const double *y, const uint16_t *mask,
+                                 int n_rates, int n_sites, int n_states, double *out,
+                                 void *stream);
+
 /* The mat_* stubs of the Price-CAT model's rate-free [site][state] GLVs
  * (linear_c.c:148-340, bound in linear.ml:28-49): the same arithmetic with one
  * rate and no -n log(n_rates) term. */

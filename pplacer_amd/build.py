@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpplacer_hip.so")
 
-HIP_SOURCES = ["pp_capi.hip"]
+HIP_SOURCES = ["pp_capi.hip", "pp_noalloc.hip"]
 CPP_SOURCES = ["pp_host.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
