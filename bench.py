@@ -258,8 +258,10 @@ def main():
         # 4 states: one FP64 site or two FP32 sites per worker lane; the FP64
         # kernel runs in the 512- and 768-lane launch classes (masked lengths
         # up to 448 and beyond), named together as MAXBD = *
-        name = ("pitch_mw_kernel<double,4,*,1,false>" if S == 4 and real == "double"
-                else "pitch_mw_kernel<float,4,512,2,false>" if S == 4
+        # (template arguments: real, rates, launch class, sites per lane,
+        # Price-CAT, FP64 site-order log sums)
+        name = ("pitch_mw_kernel<double,4,*,1,false,true>" if S == 4 and real == "double"
+                else "pitch_mw_kernel<float,4,512,2,false,false>" if S == 4
                 else "pitch_prot_kernel<double,4>")
         # per launch: the flops of the phase's launches over the sum of their
         # own durations (HIP events around each launch, on its stream); the
@@ -306,6 +308,9 @@ def main():
         "cpu_baseline": cpu,
         "forms": forms,
     }
+    if "fp32" in forms and "fp64" in forms:
+        # the batched FP32 form against the plain FP64 form, same workload
+        out["fp32_over_fp64"] = forms["fp32"]["value"] / forms["fp64"]["value"]
     if cpu:
         out["gpu_vs_cpu"] = main_f["value"] / cpu["value"]
     print(json.dumps(out), flush=True)

@@ -59,6 +59,12 @@ enum {
 /* Pendant-length priors for -p (Core.prior, core.ml:54-56) */
 enum { PP_PRIOR_FLAT_EXP = 0, PP_PRIOR_UNIFORM = 1, PP_PRIOR_INFORMATIVE = 2 };
 
+/* Precision forms of a placement pass.  PP_FP64: the reference method in
+ * FP64, each probe's per-site logs summed in site order as the stubs do.
+ * PP_FP32: stage 1 and the 4-state baseball phase on FP32 GLVs with per-site
+ * power-of-two exponents and FP64 log sums; the refinement of the kept
+ * placements (final_tolerance), their likelihoods, ratios and output in FP64.
+ * 20-state models run stage 2 in FP64 in both forms. */
 enum { PP_FP64 = 0, PP_FP32 = 1 };
 
 const char *pp_last_error(void);

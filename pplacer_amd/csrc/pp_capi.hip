@@ -2001,6 +2001,41 @@ int pp_ten_bounded_logdot(const double *x, const double *y, int first, int last,
   return twin_logdot(nullptr, x, y, nullptr, nullptr, first, last, util, n_rates, n_sites, n_states, out);
 }
 
+// Test hook, not part of the drop-in ABI: the FP64 engine's wave-parallel
+// sequential sum (wave_sequential_sum) over x[0..n), copied into LDS, beside
+// the one-lane chain.  out[0] = the wave's sum, out[1] = the chain's, out[2] =
+// the number of events (-1 where the wave fell back to the chain).
+static __global__ void seqsum_test_kernel(const double *x, int n, double *out) {
+  extern __shared__ __align__(16) double sm[];
+  double *v = sm, *acc = sm + n, *ev = sm + 2 * (size_t)n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = x[i];
+  __syncthreads();
+  int n_events = 0;
+  const double s = wave_sequential_sum(v, n, acc, ev, &n_events);
+  const double c = threadIdx.x == 0 ? ordered_sum(v, n) : 0.0;
+  if (threadIdx.x == 0) {
+    out[0] = s;
+    out[1] = c;
+    out[2] = n_events;
+  }
+}
+
+int pp_test_sequential_sum(const double *x, int n, double *out) {
+  CHECK(out && n >= 0 && (x || n == 0), PP_EINVAL, "bad arguments");
+  int dev = 0, max_lds = 65536;
+  HIPCHECK(hipGetDevice(&dev));
+  if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) max_lds = 65536;
+  const size_t lds = sizeof(double) * (2 * (size_t)n + 2 * kSeqSumEvents);
+  CHECK(lds <= (size_t)max_lds, PP_EINVAL, "n exceeds one workgroup's LDS");
+  Scratch s;
+  double *dx = s.up(x, std::max(n, 1)), *dout = s.up<double>(nullptr, 3);
+  CHECK(dx && dout, PP_ENOMEM, "device allocation failed");
+  hipLaunchKernelGGL(seqsum_test_kernel, dim3(1), dim3(64), lds, 0, dx, n, dout);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpy(out, dout, 3 * sizeof(double), hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
 // The mat_* stubs: the rate-free [site][state] GLVs of Gcat_model; the same
 // kernels with one rate and no n log R term.
 int pp_mat_log_like3(const double *statd, const double *x, const double *y, const double *z, int n_sites,

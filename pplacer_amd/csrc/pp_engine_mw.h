@@ -32,9 +32,9 @@
 // (exp(rate*bl*l_k)*x[i][k])*xit[j][k].  The FP64 form rounds every product
 // and sum separately (-ffp-contract=off) and sums the per-site logs as the
 // stub does, sequentially in masked-site order (linear_c.c:416-419): workers
-// leave each site's log in LDS and the first worker lane adds them in order
-// behind one more barrier.  The FP32
-// form uses FMAs and sums the logs (in FP64) per wave, then in wave order.
+// leave each site's log in LDS and, behind one more barrier, the first worker
+// wave reproduces that sum bit for bit (wave_sequential_sum).  The FP32 form
+// uses FMAs and sums the logs (in FP64) per wave, then in wave order.
 #pragma once
 
 #include <climits>
@@ -42,6 +42,7 @@
 
 #include "pp_engine.h"
 #include "pp_engine_ctl.h"
+#include "pp_exact_sum.h"
 
 namespace pp {
 
@@ -63,8 +64,9 @@ struct MWShared {
 // LDS of one workgroup: shared header, pendant rows [n_tsl][kMaxCodes][R*4],
 // P(prox) and P(dist) [n_sl][4][4], exponential staging [3][n_sl][4], then
 // per masked site (cap = the longest masked length the launch takes): the
-// site's log of a probe (FP64 form), its column (the masked-site list) and in
-// the streaming form its pendant row.  Gamma model:
+// site's log of a probe and the sequential sum's scratch (FP64 form), its
+// column (the masked-site list) and in the streaming form its pendant row;
+// the FP64 form also holds the sequential sum's event list.  Gamma model:
 // n_tsl = 1, n_sl = R (rate r uses slice r); Price-CAT: R = 1 and
 // n_tsl = n_sl = n_cats (a site uses its category's).
 template <typename Real, int R, bool STRICT>
@@ -80,7 +82,11 @@ struct MWLayout {
   __host__ __device__ size_t pbytes() const { return sizeof(Real) * (size_t)n_sl * 16; }
   __host__ __device__ size_t ebytes() const { return sizeof(double) * (size_t)3 * n_sl * 4; }
   __host__ __device__ size_t off_slog() const { return head + tbytes() + 2 * pbytes() + ebytes(); }
-  __host__ __device__ size_t off_sidx() const { return off_slog() + (strict ? sizeof(double) * (size_t)cap : 0); }
+  __host__ __device__ size_t off_sacc() const { return off_slog() + (strict ? sizeof(double) * (size_t)cap : 0); }
+  __host__ __device__ size_t off_sev() const { return off_sacc() + (strict ? sizeof(double) * (size_t)cap : 0); }
+  __host__ __device__ size_t off_sidx() const {
+    return off_sev() + (strict ? sizeof(double) * 2 * (size_t)kSeqSumEvents : 0);
+  }
   __host__ __device__ size_t bytes() const { return off_sidx() + sizeof(int) * (size_t)cap * (stream ? 2 : 1); }
   __device__ Real *T(char *s) const { return reinterpret_cast<Real *>(s + head); }
   __device__ Real *Pp(char *s) const { return reinterpret_cast<Real *>(s + head + tbytes()); }
@@ -89,6 +95,8 @@ struct MWLayout {
     return reinterpret_cast<double *>(s + head + tbytes() + 2 * pbytes());
   }
   __device__ double *slog(char *s) const { return reinterpret_cast<double *>(s + off_slog()); }
+  __device__ double *sacc(char *s) const { return reinterpret_cast<double *>(s + off_sacc()); }
+  __device__ double *sev(char *s) const { return reinterpret_cast<double *>(s + off_sev()); }
   __device__ int *sidx(char *s) const { return reinterpret_cast<int *>(s + off_sidx()); }
   __device__ int *sslot(char *s) const { return reinterpret_cast<int *>(s + off_sidx()) + cap; }
   static __host__ __device__ size_t align(size_t off) { return (off + 15) & ~size_t(15); }
@@ -112,38 +120,6 @@ __device__ __forceinline__ void mw_copy_model(MWShared *sh, const DevModel *dm) 
 // ---------------------------------------------------------------------------
 // worker waves
 // ---------------------------------------------------------------------------
-// 0 + v[0] + v[1] + ... + v[n-1] with one rounding per term, in order: the
-// stub's sequential log sum (linear_c.c:416-419).  A dependent chain, so it is
-// fed from LDS two blocks of eight ahead: the loads of one block are in
-// flight while the previous block is added.
-__device__ __forceinline__ double ordered_sum(const double *v, int n) {
-  double s = 0.0;
-  double a[8], b[8];
-#pragma unroll
-  for (int j = 0; j < 8; j++) a[j] = j < n ? v[j] : 0.0;
-#pragma unroll
-  for (int j = 0; j < 8; j++) b[j] = 8 + j < n ? v[8 + j] : 0.0;
-  int i = 0;
-  for (; i + 32 <= n; i += 16) {  // a = v[i..i+8), b = v[i+8..i+16), both in range
-#pragma unroll
-    for (int j = 0; j < 8; j++) s += a[j];
-#pragma unroll
-    for (int j = 0; j < 8; j++) a[j] = v[i + 16 + j];
-#pragma unroll
-    for (int j = 0; j < 8; j++) s += b[j];
-#pragma unroll
-    for (int j = 0; j < 8; j++) b[j] = v[i + 24 + j];
-  }
-  // fewer than 32 left: a and b hold v[i .. i+16) where in range
-#pragma unroll
-  for (int j = 0; j < 8; j++)
-    if (i + j < n) s += a[j];
-#pragma unroll
-  for (int j = 0; j < 8; j++)
-    if (i + 8 + j < n) s += b[j];
-  for (i += 16; i < n; i++) s += v[i];
-  return s;
-}
 
 // The cached products of masked site i live in LDS in groups of four sites:
 // byte (i/4)*kGroupBytes + t*4*sizeof(Real) + (i%4)*sizeof(Real).  Each lane's
@@ -196,7 +172,8 @@ __device__ __forceinline__ Real site_dist_util(const Real *pp, const Real *pd, c
 // CAT: Price-CAT, R == 1, site k uses P slice and pendant table sl[k] = its
 // category (site_cat); otherwise slice r for rate r and pendant table 0.
 // FP64 (strict): each active site's log goes to slog[site], and after a
-// barrier the first worker lane adds them in site order; FP32: the lane's logs
+// barrier the first worker wave adds them as the stub does, in site order
+// with one rounding per term (wave_sequential_sum); FP32: the lane's logs
 // are summed and reduced per wave.
 template <typename Real, int R, int SPL, bool CAT, bool STRICT>
 __device__ __forceinline__ void mw_worker(char *smem, const MWLayout<Real, R, STRICT> &lay, char *wl, int N,
@@ -280,7 +257,10 @@ __device__ __forceinline__ void mw_worker(char *smem, const MWLayout<Real, R, ST
           for (int k = 0; k < SPL; k++)
             if (active[k]) slog[my + k * stride] = Ops::logd(util[k]);
           __syncthreads();  // every site's log is in slog
-          if (my == 0) sh->part[0] = ordered_sum(slog, sh->M);
+          if (wv == 0) {
+            const double t = wave_sequential_sum(slog, sh->M, lay.sacc(smem), lay.sev(smem), nullptr, kSeqSumWaveMin);
+            if (lane == 0) sh->part[0] = t;
+          }
         } else {
           double lsum = 0.0;
 #pragma unroll
@@ -373,7 +353,10 @@ __device__ __forceinline__ void mw_worker_stream(char *smem, const MWLayout<Real
         }
         if constexpr (strict) {
           __syncthreads();  // every site's log is in slog
-          if (my == 0) sh->part[0] = ordered_sum(slog, M);
+          if (wv == 0) {
+            const double t = wave_sequential_sum(slog, M, lay.sacc(smem), lay.sev(smem), nullptr, kSeqSumWaveMin);
+            if (lane == 0) sh->part[0] = t;
+          }
         } else {
           lsum = wave_sum(lsum);
           if (lane == 0) sh->part[wv] = lsum;
