@@ -50,8 +50,9 @@ enum {
   PP_Q_INVALID_START = 5,  /* Minimization.InvalidStartValues */
   PP_Q_ASSERT = 6,         /* assert(prox_bl >= 0.) three_tax.ml:43 */
   PP_Q_RANK_EXHAUSTED = 7, /* query too long for the GPU path: more masked sites than a
-                              4-state stage-2 workgroup's LDS indexes (16 bytes per site
-                              of 160 KB, about 9,000); no counterpart in the reference */
+                              4-state stage-2 workgroup's LDS indexes (24 bytes per site
+                              of 160 KB in the FP64 form, about 6,500); no counterpart in
+                              the reference */
   PP_Q_MAXITER = 8,        /* Minimization.ExceededMaxIter outside the optimiser (-p) */
   PP_Q_GSL = 9             /* an uncaught Gsl_exn (-p: EBADFUNC, EBADTOL) */
 };
