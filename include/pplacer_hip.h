@@ -344,6 +344,74 @@ typedef struct {
 int pp_launch_config_get(pp_ctx *ctx, pp_launch_config *out);
 int pp_launch_config_set(pp_ctx *ctx, const pp_launch_config *cfg);
 
+/* ======================================================================= */
+/* Device GLVs: Glvm.Model's Glv for the gamma mixture model               */
+/* ======================================================================= */
+/* Gmix_model.Model.Glv (gmix_model.ml:58-257, the Glv signature of
+ * glvm.ml:28-45) with the GLV held on the GPU: a[n_rates][n_sites][n_states]
+ * doubles and e[n_sites] pulled base-2 exponents.  An OCaml Glv.t becomes a
+ * custom block around a pp_glv*, so Like_stree / Three_tax code that calls
+ * Glv.* and Model.* keeps its data in HBM between calls.  Every entry runs on
+ * the context's main stream in call order; those that return values to the
+ * host wait for them.  The model entries (pp_gmix_*) use the context's model
+ * (pp_model_build, gamma rates; a Price-CAT model gives PP_EINVAL) and need
+ * GLVs of its rates and states. */
+typedef struct pp_glv pp_glv;
+/* OCaml's fpclass, in its order: Glv.fp_classify returns the worst class */
+enum { PP_FP_NORMAL = 0, PP_FP_SUBNORMAL = 1, PP_FP_ZERO = 2, PP_FP_INFINITE = 3, PP_FP_NAN = 4 };
+
+/* Glv.make (gmix_model.ml:76-79), contents zero.  A GLV lives until
+ * pp_glv_free or until its context's pp_ctx_destroy, which frees the GLVs
+ * still made on it (their handles are invalid afterwards). */
+int pp_glv_make(pp_ctx *ctx, int n_rates, int n_sites, int n_states, pp_glv **out);
+int pp_glv_free(pp_glv *g);
+/* get_n_rates, get_n_sites, get_n_states (gmix_model.ml:63-69) */
+int pp_glv_dims(const pp_glv *g, int *n_rates, int *n_sites, int *n_states);
+/* Glv.mimic, Glv.copy, Glv.memcpy, Glv.set_unit (gmix_model.ml:82-99) */
+int pp_glv_mimic(const pp_glv *x, pp_glv **out);
+int pp_glv_copy(const pp_glv *x, pp_glv **out);
+int pp_glv_memcpy(pp_glv *dst, const pp_glv *src);
+int pp_glv_set_unit(pp_glv *g);
+/* Glv.fp_classify (gmix_model.ml:103-104, tensor.ml:43-44) -> PP_FP_* */
+int pp_glv_fp_classify(const pp_glv *g, int *cls);
+/* the GLV's contents from / to host arrays shaped as its Bigarrays
+ * (a [n_rates][n_sites][n_states], e [n_sites] OCaml ints) */
+int pp_glv_set(pp_glv *g, const double *a, const int64_t *e);
+int pp_glv_get(const pp_glv *g, double *a, int64_t *e);
+/* Glv.mask_into (gmix_model.ml:129-141): site_mask[n_mask] (n_mask = src's
+ * sites) keeps dst's number of sites, in order */
+int pp_glv_mask_into(const uint8_t *site_mask, int n_mask, const pp_glv *src, pp_glv *dst);
+/* Glv.prep_constant_rate_glv_from_lv_arr (gmix_model.ml:149-156):
+ * lv_arr [n_sites][n_states] into every rate, exponents 0 */
+int pp_glv_prep_constant_rate_from_lv_arr(pp_glv *g, const double *lv_arr);
+/* Glv.perhaps_pull_exponent (gmix_model.ml:163-187) */
+int pp_glv_perhaps_pull_exponent(int min_allowed_twoexp, pp_glv *g);
+/* Glv.masked_logdot, bounded_logdot, logdot (gmix_model.ml:194-212), the
+ * exponent terms included; the util workspace is the library's */
+int pp_glv_masked_logdot(const pp_glv *x, const pp_glv *y, const uint16_t *mask, double *out);
+int pp_glv_bounded_logdot(const pp_glv *x, const pp_glv *y, int start, int last, double *out);
+int pp_glv_logdot(const pp_glv *x, const pp_glv *y, double *out);
+/* Glv.pairwise_prod and listwise_prod (gmix_model.ml:215-233) */
+int pp_glv_pairwise_prod(pp_glv *dst, const pp_glv *x, const pp_glv *y);
+int pp_glv_listwise_prod(pp_glv *dst, const pp_glv *const *glvs, int n);
+/* Glv.summarize_post (gmix_model.ml:242-256): u[n_sites][n_states] = the
+ * per-site state sums over rates; the caller applies summarize_f */
+int pp_glv_summarize_post(const pp_glv *g, double *u);
+
+/* Model.log_like3 (gmix_model.ml:298-304) */
+int pp_gmix_log_like3(pp_ctx *ctx, const pp_glv *x, const pp_glv *y, const pp_glv *z, double *out);
+/* Model.site_log_like_arr3 (gmix_model.ml:323-345): out[n_sites];
+ * PP_EQUERY ("Site %d has zero likelihood.") where a site's is not positive */
+int pp_gmix_site_log_like_arr3(pp_ctx *ctx, const pp_glv *x, const pp_glv *y, const pp_glv *z,
+                               double *out);
+/* Model.slow_log_like3 (gmix_model.ml:347-348) */
+int pp_gmix_slow_log_like3(pp_ctx *ctx, const pp_glv *x, const pp_glv *y, const pp_glv *z, double *out);
+/* Model.evolve_into (gmix_model.ml:308-315): dst = src evolved for branch
+ * length bl by the model's P(bl) of every rate (Diagd.multi_exp) */
+int pp_gmix_evolve_into(pp_ctx *ctx, pp_glv *dst, const pp_glv *src, double bl);
+/* Model.statd_pairwise_prod (gmix_model.ml:319-321) */
+int pp_gmix_statd_pairwise_prod(pp_ctx *ctx, pp_glv *dst, const pp_glv *a, const pp_glv *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,6 +114,31 @@ SIGNATURES = {
     "pp_synchronize": (_i, [_vp]),
     "pp_launch_config_get": (_i, [_vp, C.POINTER(LaunchConfig)]),
     "pp_launch_config_set": (_i, [_vp, C.POINTER(LaunchConfig)]),
+    # device GLVs (Glvm.Model's Glv of Gmix_model)
+    "pp_glv_make": (_i, [_vp, _i, _i, _i, C.POINTER(_vp)]),
+    "pp_glv_free": (_i, [_vp]),
+    "pp_glv_dims": (_i, [_vp, _ip, _ip, _ip]),
+    "pp_glv_mimic": (_i, [_vp, C.POINTER(_vp)]),
+    "pp_glv_copy": (_i, [_vp, C.POINTER(_vp)]),
+    "pp_glv_memcpy": (_i, [_vp, _vp]),
+    "pp_glv_set_unit": (_i, [_vp]),
+    "pp_glv_fp_classify": (_i, [_vp, _ip]),
+    "pp_glv_set": (_i, [_vp, _dp, _i64p]),
+    "pp_glv_get": (_i, [_vp, _dp, _i64p]),
+    "pp_glv_mask_into": (_i, [_u8p, _i, _vp, _vp]),
+    "pp_glv_prep_constant_rate_from_lv_arr": (_i, [_vp, _dp]),
+    "pp_glv_perhaps_pull_exponent": (_i, [_i, _vp]),
+    "pp_glv_masked_logdot": (_i, [_vp, _vp, _u16p, _dp]),
+    "pp_glv_bounded_logdot": (_i, [_vp, _vp, _i, _i, _dp]),
+    "pp_glv_logdot": (_i, [_vp, _vp, _dp]),
+    "pp_glv_pairwise_prod": (_i, [_vp, _vp, _vp]),
+    "pp_glv_listwise_prod": (_i, [_vp, C.POINTER(_vp), _i]),
+    "pp_glv_summarize_post": (_i, [_vp, _dp]),
+    "pp_gmix_log_like3": (_i, [_vp, _vp, _vp, _vp, _dp]),
+    "pp_gmix_site_log_like_arr3": (_i, [_vp, _vp, _vp, _vp, _dp]),
+    "pp_gmix_slow_log_like3": (_i, [_vp, _vp, _vp, _vp, _dp]),
+    "pp_gmix_evolve_into": (_i, [_vp, _vp, _vp, _d]),
+    "pp_gmix_statd_pairwise_prod": (_i, [_vp, _vp, _vp, _vp]),
 }
 
 _lib = None

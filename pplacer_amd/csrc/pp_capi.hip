@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -133,6 +134,8 @@ struct pp_ctx {
   // results staging (sorted order)
   std::vector<pp_placement> h_out;
   std::vector<int> h_nout, h_status;
+  // device GLVs made on this context (pp_glv_make); freed with it
+  std::set<pp_glv *> glvs;
   int K_last = 0;
 };
 
@@ -1462,6 +1465,7 @@ int pp_ctx_destroy(pp_ctx *c) {
   if (c->s1) (void)hipStreamSynchronize(c->s1);
   for (hipStream_t b : c->bs)
     if (b) (void)hipStreamSynchronize(b);
+  while (!c->glvs.empty()) pp_glv_free(*c->glvs.begin());
   DBuf *bufs[] = {&c->b_model, &c->b_bl, &c->b_d64, &c->b_p64, &c->b_d32, &c->b_p32, &c->b_de,
                   &c->b_pe, &c->b_s64, &c->b_se, &c->b_de32, &c->b_pe32, &c->b_lt, &c->b_scratch, &c->b_cat, &c->b_prior, &c->b_codes, &c->b_qM, &c->b_tcodes,
                   &c->b_tile_lo, &c->b_tile_hi, &c->b_tile_q, &c->b_ttab, &c->b_scores, &c->b_res,
@@ -2066,3 +2070,5 @@ int pp_mat_bounded_logdot(const double *x, const double *y, int first, int last,
 }
 
 }  // extern "C"
+
+#include "pp_glvm.hip"

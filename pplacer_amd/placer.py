@@ -125,6 +125,8 @@ class Placer:
 
     def close(self):
         if self._ctx:
+            from . import glvm
+            glvm.forget_context(self._ctx)  # pp_ctx_destroy frees the context's GLVs
             lib().pp_ctx_destroy(self._ctx)
             self._ctx = C.c_void_p()
 
