@@ -52,6 +52,11 @@ def pitch_flops(stats, R=4, S=4):
             + (4 * R * S * S + 4 * R * S) * stats.n_site_probes_dist_pitch)
 
 
+def _per(x, ms):
+    """x per second over ms milliseconds (None without a time)."""
+    return x / (ms * 1e-3) if ms > 0 else None
+
+
 def read_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size)."""
@@ -156,6 +161,9 @@ def main():
     ap.add_argument("--tune", action="store_true",
                     help="tune each form's launch shape (bucket streams per stage-2 phase, chunk "
                          "size) on the resident queries before its warmup (untimed)")
+    ap.add_argument("--graph", action="store_true",
+                    help="capture each form's placement pass into a HIP graph and replay it "
+                         "(no per-phase or per-launch times are recorded then)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size "
                          "kernel for scaling hardware byte counters)")
@@ -185,6 +193,8 @@ def main():
                                                                   wl.tree.n_nodes, N))
     pl.upload_queries(queries)
     prefs = placer.default_prefs(calc_pp=args.calc_pp)
+    if args.graph:
+        pl.set_launch_config(graph=1)
     if args.copy_calibration_mb:
         copy_calibration(args.copy_calibration_mb << 20)
         log("copy calibration: %d MiB read + %d MiB written" % (args.copy_calibration_mb,
@@ -227,11 +237,13 @@ def main():
             "queries_per_s": args.queries / (ms / 1000.0),
             "t_table_ms": st.t_table_ms, "t_stage1_ms": st.t_stage1_ms, "t_pitch_ms": st.t_pitch_ms,
             "t_refine_ms": st.t_refine_ms, "t_pp_ms": st.t_pp_ms,
-            "stage1_ref_equiv_tflops": s1_ref_flops / (st.t_stage1_ms * 1e-3) / 1e12,
-            "stage1_gather_GBps": s1_gather_bytes / (st.t_stage1_ms * 1e-3) / 1e9,
-            "stage2_tflops": s2_flops / ((st.t_pitch_ms + st.t_refine_ms) * 1e-3) / 1e12,
-            "pitch_u2_tflops": pitch_flops(st, S=S) / (st.t_pitch_ms * 1e-3) / 1e12,
-            "pitch_u2_tflops_per_launch": pitch_flops(st, S=S) / (st.t_pitch_kernel_ms * 1e-3) / 1e12,
+            # (graph mode records no per-phase times: these read null)
+            "stage1_ref_equiv_tflops": _per(s1_ref_flops / 1e12, st.t_stage1_ms),
+            "stage1_gather_GBps": _per(s1_gather_bytes / 1e9, st.t_stage1_ms),
+            "stage2_tflops": _per(s2_flops / 1e12, st.t_pitch_ms + st.t_refine_ms),
+            "pitch_u2_tflops": _per(pitch_flops(st, S=S) / 1e12, st.t_pitch_ms),
+            "pitch_u2_tflops_per_launch": _per(pitch_flops(st, S=S) / 1e12, st.t_pitch_kernel_ms),
+            "t_graph_ms": st.t_ml_ms if args.graph else None,
             "pitch_launches": st.n_pitch_launches,
             "pitch_avg_launch_ms": st.t_pitch_kernel_ms / max(1, st.n_pitch_launches),
             "pitch_phase_ms_per_launch": st.t_pitch_ms / max(1, st.n_pitch_launches),
@@ -244,13 +256,15 @@ def main():
     # dominant kernel: the one with the largest device time
     kern = max([("stage1_kernel", st_main["t_stage1_ms"]), ("pitch_kernel", st_main["t_pitch_ms"]),
                 ("refine_kernels", st_main["t_refine_ms"])], key=lambda kv: kv[1])[0]
+    if args.graph:  # no phase times in graph mode
+        kern = "pitch_kernel"
     peak = FP64_PEAK_TFLOPS if (args.precision == "fp64" or S == 20) else FP32_PEAK_TFLOPS
     real = "double" if args.precision == "fp64" else "float"
     if kern == "stage1_kernel":
         name = "stage1_kernel<%s,16>" % real
         roofline = {"bound": "hbm", "kernel": name,
                     "achieved": st_main["stage1_gather_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": st_main["stage1_gather_GBps"] / HBM_PEAK_GBS,
+                    "frac": (st_main["stage1_gather_GBps"] or 0) / HBM_PEAK_GBS,
                     "traffic": read_traffic(name),
                     "note": "gathered table bytes; most are served by L1/L2"}
     else:
@@ -274,11 +288,11 @@ def main():
                     "pipe": ("vector ALU %s (4-state contractions do not use MFMA)" % args.precision
                              if S == 4 else "vector ALU fp64 (eigenbasis evolves)"),
                     "kernel": name, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                    "frac": achieved / peak, "traffic": read_traffic(name),
+                    "frac": achieved / peak if achieved else None, "traffic": read_traffic(name),
                     "avg_launch_ms": st_main["pitch_avg_launch_ms"],
                     "launches": st_main["pitch_launches"],
                     "phase_achieved": st_main["pitch_u2_tflops"],
-                    "phase_frac": st_main["pitch_u2_tflops"] / peak,
+                    "phase_frac": st_main["pitch_u2_tflops"] / peak if st_main["pitch_u2_tflops"] else None,
                     "work": "reference U2 flops of the pitched probes: (2RS^2+4RS) per masked site "
                             "of a pendant probe, (4RS^2+4RS) of a distal probe"}
         if kern != "pitch_kernel":

@@ -332,6 +332,12 @@ int pp_synchronize(pp_ctx *ctx);
  *   chunk: queries per device chunk (stage 1 of the next chunk overlaps
  *     stage 2 of the current one), 1 .. PP_MAX_CHUNK, further capped so that a
  *     chunk's scores ([chunk][n_edges] doubles) stay under 2 GiB.
+ *   graph: 1 captures the pass's stage-1 and stage-2 launches (every chunk,
+ *     every stream) into a HIP graph and replays it on the next passes with
+ *     the same launch plan (resident queries, prefs, precision, shape and
+ *     buffers); a pass with another plan captures again.  With graph = 1
+ *     pp_stats holds the whole graph's device time in t_ml_ms and no
+ *     per-phase or per-launch times.  0 (default) launches eagerly.
  * pp_launch_config_set returns PP_EINVAL, changing nothing, for a value out
  * of range. */
 #define PP_MAX_BUCKET_STREAMS 8
@@ -340,6 +346,7 @@ typedef struct {
   int pitch_streams[2];
   int refine_streams[2];
   int chunk;
+  int graph;
 } pp_launch_config;
 int pp_launch_config_get(pp_ctx *ctx, pp_launch_config *out);
 int pp_launch_config_set(pp_ctx *ctx, const pp_launch_config *cfg);

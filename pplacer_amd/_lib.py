@@ -54,7 +54,7 @@ class Stats(C.Structure):
 
 class LaunchConfig(C.Structure):
     _fields_ = [("pitch_streams", C.c_int * 2), ("refine_streams", C.c_int * 2),
-                ("chunk", C.c_int)]
+                ("chunk", C.c_int), ("graph", C.c_int)]
 
 
 PP_MAX_BUCKET_STREAMS, PP_MAX_CHUNK = 8, 262144

@@ -96,7 +96,15 @@ struct pp_ctx {
   hipStream_t s1 = nullptr;           // stage 1 of the next chunk
   hipStream_t bs[kBucketStreams] = {};  // concurrent stage-2 buckets
   hipStream_t cur = nullptr;          // the stream the launch helpers use
-  pp_launch_config launch{{kPitchStreams[0], kPitchStreams[1]}, {kRefineStreams[0], kRefineStreams[1]}, kChunk};
+  pp_launch_config launch{{kPitchStreams[0], kPitchStreams[1]}, {kRefineStreams[0], kRefineStreams[1]}, kChunk, 0};
+  // graph mode: captured chunk loops by plan key (most recent last; a pass
+  // and the FP64 re-placement of its FP32 failures keep one each)
+  struct CapturedPass {
+    uint64_t key;
+    hipGraphExec_t exec;
+    long long launches, pitch_launches;
+  };
+  std::vector<CapturedPass> graphs;
   HostModel model;
   DBuf b_model;
   DevModel *d_model = nullptr;
@@ -919,6 +927,26 @@ struct EventPool {
   }
 };
 
+// FNV-1a over the bytes of the values a pass's launches derive from (the key
+// of its captured graph).
+struct PlanKey {
+  uint64_t v = 1469598103934665603ull;
+  void bytes(const void *p, size_t n) {
+    const unsigned char *b = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < n; i++) {
+      v ^= b[i];
+      v *= 1099511628211ull;
+    }
+  }
+  template <class T>
+  void add(const T &x) { bytes(&x, sizeof(T)); }
+  template <class T>
+  void add_vec(const std::vector<T> &x) {
+    add(x.size());
+    if (!x.empty()) bytes(x.data(), x.size() * sizeof(T));
+  }
+};
+
 // Restores the context's launch stream on every exit path.
 struct LaunchStream {
   pp_ctx *c;
@@ -1088,121 +1116,140 @@ int place_pass(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats)
     }
     return PP_OK;
   };
-  for (int ci = 0; ci < n_chunks; ci++) {
-    const int q0 = cq0[ci], nq = cq0[ci + 1] - q0;
-    double *scores = d_scores + (size_t)(ci % n_sbuf) * chunk * E;
-    Marks &mk = marks[ci];
-    for (hipEvent_t *e : {&mk.s1a, &mk.s1b, &mk.p0, &mk.p1, &mk.r1, &mk.m1, &mk.f1}) {
-      *e = pool.get();
-      CHECK(*e, PP_EDEVICE, "hipEventCreate failed");
+  // The chunk loop: launched eagerly, or (graph mode) captured once into a
+  // graph that later passes with the same plan replay.  The key covers
+  // everything the launches' arguments and shapes derive from.
+  const bool graph = c->launch.graph != 0;
+  uint64_t key = 0;
+  if (graph) {
+    PlanKey h;
+    h.add(Q); h.add(K); h.add(precision); h.add(*prefs); h.add(c->launch); h.add(n_chunks);
+    h.add_vec(cq0); h.add_vec(t_off);
+    for (const auto &bo : boffs) h.add_vec(bo);
+    const void *ptrs[] = {d_out, d_nout, d_status, d_scores, d_res, d_nres, d_kept, d_nkeep, d_qlist,
+                          d_counter, d_stats, d_tlo, d_thi, d_tq, c->b_tcodes.p, c->b_codes.p, c->b_qM.p,
+                          c->b_lt.p, c->b_se.p, c->b_d64.p, c->b_p64.p, c->b_d32.p, c->b_p32.p, c->b_de.p,
+                          c->b_pe.p, c->b_de32.p, c->b_pe32.p, c->b_bl.p, c->b_sscr.p, c->d_model, c->b_cat.p,
+                          c->b_prior.p};
+    for (const void *ptr : ptrs) h.add(ptr);
+    h.add(c->reg_limit); h.add(c->long_cap); h.add(c->sscr_blocks);
+    key = h.v;
+  }
+  hipEvent_t g0 = pool.get(), g1 = pool.get();
+  CHECK(g0 && g1, PP_EDEVICE, "hipEventCreate failed");
+  if (graph) HIPCHECK(hipEventRecord(g0, c->stream));
+  constexpr size_t kMaxGraphs = 4;
+  auto cached = std::find_if(c->graphs.begin(), c->graphs.end(), [&](const auto &g) { return g.key == key; });
+  if (graph && cached != c->graphs.end()) {
+    HIPCHECK(hipGraphLaunch(cached->exec, c->stream));
+    launches += cached->launches;
+    pitch_launches += cached->pitch_launches;
+  } else {
+    const long long l0 = launches, pl0 = pitch_launches;
+    if (graph) {
+      if (c->graphs.size() >= kMaxGraphs) {
+        (void)hipGraphExecDestroy(c->graphs.front().exec);
+        c->graphs.erase(c->graphs.begin());
+      }
+      HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+      // the stage-1 stream joins the capture
+      hipEvent_t ecap = pool.get();
+      int cst = ecap ? PP_OK : PP_EDEVICE;
+      if (!cst && hipEventRecord(ecap, c->stream) != hipSuccess) cst = PP_EDEVICE;
+      if (!cst && hipStreamWaitEvent(c->s1, ecap, 0) != hipSuccess) cst = PP_EDEVICE;
+      if (cst) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        set_error("graph capture could not fork the stage-1 stream");
+        return cst;
+      }
     }
-    // stage 1 on its stream, once the baseball phase two chunks back has
-    // finished with this score buffer
-    if (scores_free[ci % n_sbuf]) HIPCHECK(hipStreamWaitEvent(c->s1, scores_free[ci % n_sbuf], 0));
-    ls.set(c->s1);
-    HIPCHECK(hipEventRecord(mk.s1a, c->s1));
-    const int to = t_off[ci];
-    st = fp32 ? launch_stage1_tiles<float>(c, q0, nq, scores, d_tlo + to, d_thi + to, d_tq + (size_t)to * kTQ)
-              : launch_stage1_tiles<double>(c, q0, nq, scores, d_tlo + to, d_thi + to, d_tq + (size_t)to * kTQ);
-    if (st) return st;
-    launches += 2;
-    HIPCHECK(hipEventRecord(mk.s1b, c->s1));
-    ls.set(c->stream);
-    const std::vector<int> &boff = boffs[ci];
-    int *qlist = d_qlist + q0;
-    HIPCHECK(hipMemsetAsync(d_counter, 0, 128 * 4, c->stream));
-    HIPCHECK(hipStreamWaitEvent(c->stream, mk.s1b, 0));
-    HIPCHECK(hipEventRecord(mk.p0, c->stream));
-    const uint8_t *codes = reinterpret_cast<const uint8_t *>(c->b_codes.p) + (size_t)q0 * N;
-    int *status = d_status + q0;
-    st = phase(boff, pitch_streams, [&](int b) -> int {
-      PitchArgs p{};
-      p.model = c->d_model;
-      p.codes = codes;
-      p.scores = scores;
-      p.qlist = qlist + boff[b];
-      p.nq = boff[b + 1] - boff[b];
-      p.counter = d_counter + b;
-      p.darr = s2_fp32 ? c->b_d32.p : c->b_d64.p;
-      p.parr = s2_fp32 ? c->b_p32.p : c->b_p64.p;
-      p.darr_e = reinterpret_cast<const int *>(s2_fp32 ? c->b_de32.p : c->b_de.p);
-      p.parr_e = reinterpret_cast<const int *>(s2_fp32 ? c->b_pe32.p : c->b_pe.p);
-      p.bl = reinterpret_cast<const double *>(c->b_bl.p);
-      p.N = N;
-      p.E = E;
-      p.start_pend = prefs->start_pend;
-      p.max_pend = prefs->max_pend;
-      p.initial_tol = prefs->initial_tolerance;
-      p.strike_box = prefs->strike_box;
-      p.t_max_pitches = t_max_pitches;
-      p.t_max_strikes = t_max_strikes;
-      p.cap = cap;
-      p.res = d_res;
-      p.n_res = d_nres;
-      p.status = status;
-      p.n_calls = d_stats;
-      p.n_pitched = d_stats + 1;
-      launches++;
-      pitch_launches++;
-      hipEvent_t k0 = pool.get(), k1 = pool.get();
-      CHECK(k0 && k1, PP_EDEVICE, "hipEventCreate failed");
-      HIPCHECK(hipEventRecord(k0, c->cur));
-      const int r = s2_fp32 ? dispatch_pitch<float, false>(c, p, b) : dispatch_pitch<double, true>(c, p, b);
-      HIPCHECK(hipEventRecord(k1, c->cur));
-      pitch_ev.emplace_back(k0, k1);
-      return r;
-    });
-    if (st) return st;
-    HIPCHECK(hipEventRecord(mk.p1, c->stream));
-    scores_free[ci % n_sbuf] = mk.p1;
-    KeepArgs ka;
-    ka.Q = nq;
-    ka.cap = cap;
-    ka.K = K;
-    ka.keep = prefs->keep_at_most;
-    ka.res = d_res;
-    ka.n_res = d_nres;
-    ka.status = status;
-    ka.log_keep_factor = std::log(prefs->keep_factor);
-    ka.kept = d_kept;
-    ka.n_keep = d_nkeep;
-    hipLaunchKernelGGL(keep_kernel, dim3((nq + 255) / 256), dim3(256), 0, c->stream, ka);
-    HIPCHECK(hipGetLastError());
-    st = phase(boff, refine_streams, [&](int b) -> int {
-      OptArgs o{};
-      std::memset(&o, 0, sizeof(o));
-      o.model = c->d_model;
-      o.codes = codes;
-      o.N = N;
-      o.mode = 1;
-      o.n_jobs = (boff[b + 1] - boff[b]) * K;
-      o.K = K;
-      o.qlist = qlist + boff[b];
-      o.counter = d_counter + 32 + b;
-      o.darr = c->b_d64.p;
-      o.parr = c->b_p64.p;
-      o.darr_e = reinterpret_cast<const int *>(c->b_de.p);
-      o.parr_e = reinterpret_cast<const int *>(c->b_pe.p);
-      o.bl = reinterpret_cast<const double *>(c->b_bl.p);
-      o.tol = 1e-5;  // core.ml:14 final_tolerance
-      o.max_pend = prefs->max_pend;
-      o.start_pend = prefs->start_pend;
-      o.kept = d_kept;
-      o.n_keep = d_nkeep;
-      o.status = status;
-      o.n_calls = d_stats;
-      launches++;
-      // the FP64 form sums site logs in site order; the FP32 form's FP64
-      // refinement sums them per wave
-      return fp32 ? dispatch_opt<double, false>(c, o, b) : dispatch_opt<double, true>(c, o, b);
-    });
-    if (st) return st;
-    HIPCHECK(hipEventRecord(mk.r1, c->stream));
-    if (prefs->calc_pp) {  // core.ml:378-392, one job per kept placement
-      // on the main stream: forked over the refinement's streams, this phase
-      // measured 196 -> 215-221 ms (FP64) at 20,000 queries
-      st = phase(boff, 0, [&](int b) -> int {
-        OptArgs o = marg_proto;
+    auto run_chunks = [&]() -> int {
+    for (int ci = 0; ci < n_chunks; ci++) {
+      const int q0 = cq0[ci], nq = cq0[ci + 1] - q0;
+      double *scores = d_scores + (size_t)(ci % n_sbuf) * chunk * E;
+      Marks &mk = marks[ci];
+      for (hipEvent_t *e : {&mk.s1a, &mk.s1b, &mk.p0, &mk.p1, &mk.r1, &mk.m1, &mk.f1}) {
+        *e = pool.get();
+        CHECK(*e, PP_EDEVICE, "hipEventCreate failed");
+      }
+      // stage 1 on its stream, once the baseball phase two chunks back has
+      // finished with this score buffer
+      if (scores_free[ci % n_sbuf]) HIPCHECK(hipStreamWaitEvent(c->s1, scores_free[ci % n_sbuf], 0));
+      ls.set(c->s1);
+      if (!graph) HIPCHECK(hipEventRecord(mk.s1a, c->s1));
+      const int to = t_off[ci];
+      st = fp32 ? launch_stage1_tiles<float>(c, q0, nq, scores, d_tlo + to, d_thi + to, d_tq + (size_t)to * kTQ)
+                : launch_stage1_tiles<double>(c, q0, nq, scores, d_tlo + to, d_thi + to, d_tq + (size_t)to * kTQ);
+      if (st) return st;
+      launches += 2;
+      HIPCHECK(hipEventRecord(mk.s1b, c->s1));
+      ls.set(c->stream);
+      const std::vector<int> &boff = boffs[ci];
+      int *qlist = d_qlist + q0;
+      hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(128), 0, c->stream, d_counter, 128);
+      HIPCHECK(hipStreamWaitEvent(c->stream, mk.s1b, 0));
+      if (!graph) HIPCHECK(hipEventRecord(mk.p0, c->stream));
+      const uint8_t *codes = reinterpret_cast<const uint8_t *>(c->b_codes.p) + (size_t)q0 * N;
+      int *status = d_status + q0;
+      st = phase(boff, pitch_streams, [&](int b) -> int {
+        PitchArgs p{};
+        p.model = c->d_model;
+        p.codes = codes;
+        p.scores = scores;
+        p.qlist = qlist + boff[b];
+        p.nq = boff[b + 1] - boff[b];
+        p.counter = d_counter + b;
+        p.darr = s2_fp32 ? c->b_d32.p : c->b_d64.p;
+        p.parr = s2_fp32 ? c->b_p32.p : c->b_p64.p;
+        p.darr_e = reinterpret_cast<const int *>(s2_fp32 ? c->b_de32.p : c->b_de.p);
+        p.parr_e = reinterpret_cast<const int *>(s2_fp32 ? c->b_pe32.p : c->b_pe.p);
+        p.bl = reinterpret_cast<const double *>(c->b_bl.p);
+        p.N = N;
+        p.E = E;
+        p.start_pend = prefs->start_pend;
+        p.max_pend = prefs->max_pend;
+        p.initial_tol = prefs->initial_tolerance;
+        p.strike_box = prefs->strike_box;
+        p.t_max_pitches = t_max_pitches;
+        p.t_max_strikes = t_max_strikes;
+        p.cap = cap;
+        p.res = d_res;
+        p.n_res = d_nres;
+        p.status = status;
+        p.n_calls = d_stats;
+        p.n_pitched = d_stats + 1;
+        launches++;
+        pitch_launches++;
+        if (graph) return s2_fp32 ? dispatch_pitch<float, false>(c, p, b) : dispatch_pitch<double, true>(c, p, b);
+        hipEvent_t k0 = pool.get(), k1 = pool.get();
+        CHECK(k0 && k1, PP_EDEVICE, "hipEventCreate failed");
+        HIPCHECK(hipEventRecord(k0, c->cur));
+        const int r = s2_fp32 ? dispatch_pitch<float, false>(c, p, b) : dispatch_pitch<double, true>(c, p, b);
+        HIPCHECK(hipEventRecord(k1, c->cur));
+        pitch_ev.emplace_back(k0, k1);
+        return r;
+      });
+      if (st) return st;
+      HIPCHECK(hipEventRecord(mk.p1, c->stream));
+      scores_free[ci % n_sbuf] = mk.p1;
+      KeepArgs ka;
+      ka.Q = nq;
+      ka.cap = cap;
+      ka.K = K;
+      ka.keep = prefs->keep_at_most;
+      ka.res = d_res;
+      ka.n_res = d_nres;
+      ka.status = status;
+      ka.log_keep_factor = std::log(prefs->keep_factor);
+      ka.kept = d_kept;
+      ka.n_keep = d_nkeep;
+      hipLaunchKernelGGL(keep_kernel, dim3((nq + 255) / 256), dim3(256), 0, c->stream, ka);
+      HIPCHECK(hipGetLastError());
+      st = phase(boff, refine_streams, [&](int b) -> int {
+        OptArgs o{};
+        std::memset(&o, 0, sizeof(o));
         o.model = c->d_model;
         o.codes = codes;
         o.N = N;
@@ -1210,44 +1257,107 @@ int place_pass(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats)
         o.n_jobs = (boff[b + 1] - boff[b]) * K;
         o.K = K;
         o.qlist = qlist + boff[b];
-        o.counter = d_counter + 64 + b;
+        o.counter = d_counter + 32 + b;
         o.darr = c->b_d64.p;
         o.parr = c->b_p64.p;
         o.darr_e = reinterpret_cast<const int *>(c->b_de.p);
         o.parr_e = reinterpret_cast<const int *>(c->b_pe.p);
         o.bl = reinterpret_cast<const double *>(c->b_bl.p);
+        o.tol = 1e-5;  // core.ml:14 final_tolerance
         o.max_pend = prefs->max_pend;
-        o.pp_rel_err = prefs->pp_rel_err;
+        o.start_pend = prefs->start_pend;
         o.kept = d_kept;
         o.n_keep = d_nkeep;
         o.status = status;
         o.n_calls = d_stats;
         launches++;
-        return fp32 ? dispatch_marg<false>(c, o, b) : dispatch_marg<true>(c, o, b);
+        // the FP64 form sums site logs in site order; the FP32 form's FP64
+        // refinement sums them per wave
+        return fp32 ? dispatch_opt<double, false>(c, o, b) : dispatch_opt<double, true>(c, o, b);
       });
       if (st) return st;
+      if (!graph) HIPCHECK(hipEventRecord(mk.r1, c->stream));
+      if (prefs->calc_pp) {  // core.ml:378-392, one job per kept placement
+        // on the main stream: forked over the refinement's streams, this phase
+        // measured 196 -> 215-221 ms (FP64) at 20,000 queries
+        st = phase(boff, 0, [&](int b) -> int {
+          OptArgs o = marg_proto;
+          o.model = c->d_model;
+          o.codes = codes;
+          o.N = N;
+          o.mode = 1;
+          o.n_jobs = (boff[b + 1] - boff[b]) * K;
+          o.K = K;
+          o.qlist = qlist + boff[b];
+          o.counter = d_counter + 64 + b;
+          o.darr = c->b_d64.p;
+          o.parr = c->b_p64.p;
+          o.darr_e = reinterpret_cast<const int *>(c->b_de.p);
+          o.parr_e = reinterpret_cast<const int *>(c->b_pe.p);
+          o.bl = reinterpret_cast<const double *>(c->b_bl.p);
+          o.max_pend = prefs->max_pend;
+          o.pp_rel_err = prefs->pp_rel_err;
+          o.kept = d_kept;
+          o.n_keep = d_nkeep;
+          o.status = status;
+          o.n_calls = d_stats;
+          launches++;
+          return fp32 ? dispatch_marg<false>(c, o, b) : dispatch_marg<true>(c, o, b);
+        });
+        if (st) return st;
+      }
+      if (!graph) HIPCHECK(hipEventRecord(mk.m1, c->stream));
+      FinalArgs fa;
+      fa.Q = nq;
+      fa.K = K;
+      fa.calc_pp = prefs->calc_pp;
+      fa.kept = d_kept;
+      fa.n_keep = d_nkeep;
+      fa.status = status;
+      fa.out = d_out + (size_t)q0 * K;
+      fa.n_out = d_nout + q0;
+      hipLaunchKernelGGL(finalize_kernel, dim3((nq + 255) / 256), dim3(256), 0, c->stream, fa);
+      HIPCHECK(hipGetLastError());
+      launches += 2;
+      if (!graph) HIPCHECK(hipEventRecord(mk.f1, c->stream));
     }
-    HIPCHECK(hipEventRecord(mk.m1, c->stream));
-    FinalArgs fa;
-    fa.Q = nq;
-    fa.K = K;
-    fa.calc_pp = prefs->calc_pp;
-    fa.kept = d_kept;
-    fa.n_keep = d_nkeep;
-    fa.status = status;
-    fa.out = d_out + (size_t)q0 * K;
-    fa.n_out = d_nout + q0;
-    hipLaunchKernelGGL(finalize_kernel, dim3((nq + 255) / 256), dim3(256), 0, c->stream, fa);
-    HIPCHECK(hipGetLastError());
-    launches += 2;
-    HIPCHECK(hipEventRecord(mk.f1, c->stream));
+      return PP_OK;
+    };
+    st = run_chunks();
+    if (graph) {
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+      if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      if (ec != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        set_error(std::string("hipStreamEndCapture failed: ") + hipGetErrorString(ec));
+        return PP_EDEVICE;
+      }
+      hipGraphExec_t exec = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess || !exec) {
+        set_error(std::string("hipGraphInstantiate failed: ") + hipGetErrorString(ei));
+        return PP_EDEVICE;
+      }
+      c->graphs.push_back({key, exec, launches - l0, pitch_launches - pl0});
+      HIPCHECK(hipGraphLaunch(exec, c->stream));
+    } else if (st) {
+      return st;
+    }
   }
+  if (graph) HIPCHECK(hipEventRecord(g1, c->stream));
   HIPCHECK(hipStreamSynchronize(c->s1));
   HIPCHECK(hipStreamSynchronize(c->stream));
-  float t_table = 0;
+  float t_table = 0, t_graph = 0;
   double t_s1 = 0, t_pitch = 0, t_ref = 0, t_pp = 0;
   HIPCHECK(hipEventElapsedTime(&t_table, e_t0, e_t1));
+  if (graph) HIPCHECK(hipEventElapsedTime(&t_graph, g0, g1));
   for (const Marks &mk : marks) {
+    if (graph) break;  // no per-phase events in graph mode
     float a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0;
     HIPCHECK(hipEventElapsedTime(&a1, mk.s1a, mk.s1b));
     HIPCHECK(hipEventElapsedTime(&a2, mk.p0, mk.p1));
@@ -1273,7 +1383,7 @@ int place_pass(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats)
     stats->t_pitch_ms = t_pitch;
     stats->t_refine_ms = t_ref;
     stats->t_ranking_ms = t_table + t_s1;
-    stats->t_ml_ms = t_pitch + t_ref;
+    stats->t_ml_ms = graph ? t_graph : t_pitch + t_ref;
     stats->n_logdots = 0;
     stats->n_site_pairs = 0;
     for (int q = 0; q < Q; q++)
@@ -1466,6 +1576,7 @@ int pp_ctx_destroy(pp_ctx *c) {
   for (hipStream_t b : c->bs)
     if (b) (void)hipStreamSynchronize(b);
   while (!c->glvs.empty()) pp_glv_free(*c->glvs.begin());
+  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
   DBuf *bufs[] = {&c->b_model, &c->b_bl, &c->b_d64, &c->b_p64, &c->b_d32, &c->b_p32, &c->b_de,
                   &c->b_pe, &c->b_s64, &c->b_se, &c->b_de32, &c->b_pe32, &c->b_lt, &c->b_scratch, &c->b_cat, &c->b_prior, &c->b_codes, &c->b_qM, &c->b_tcodes,
                   &c->b_tile_lo, &c->b_tile_hi, &c->b_tile_q, &c->b_ttab, &c->b_scores, &c->b_res,
@@ -1676,6 +1787,7 @@ int pp_launch_config_set(pp_ctx *c, const pp_launch_config *cfg) {
           "refine_streams must be in 0..PP_MAX_BUCKET_STREAMS");
   }
   CHECK(cfg->chunk >= 1 && cfg->chunk <= PP_MAX_CHUNK, PP_EINVAL, "chunk must be in 1..PP_MAX_CHUNK");
+  CHECK(cfg->graph == 0 || cfg->graph == 1, PP_EINVAL, "graph must be 0 or 1");
   c->launch = *cfg;
   return PP_OK;
 }

@@ -533,6 +533,12 @@ __global__ __launch_bounds__(512) void marginal_prot_kernel(OptArgs a) {
 // descending, keeping i < keep_at_most with ll >= best + log keep_factor.
 // One thread per query; only the first keep_at_most ranks are needed.
 // ---------------------------------------------------------------------------
+// n ints <- 0: work-queue counters, cleared by a kernel so that a captured
+// graph holds no memset node
+__global__ void zero_ints_kernel(int *p, int n) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) p[t] = 0;
+}
+
 __global__ void keep_kernel(KeepArgs a) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= a.Q) return;

@@ -316,13 +316,14 @@ class Placer:
 
     def launch_config(self) -> dict:
         """The pass's launch shape: streams per stage-2 phase and form
-        (index 0 = FP64, 1 = FP32) and queries per device chunk."""
+        (index 0 = FP64, 1 = FP32), queries per device chunk, and whether the
+        pass's launches are captured into a replayed HIP graph."""
         cfg = LaunchConfig()
         check(lib().pp_launch_config_get(self._ctx, C.byref(cfg)), "launch_config_get")
         return {"pitch_streams": list(cfg.pitch_streams), "refine_streams": list(cfg.refine_streams),
-                "chunk": cfg.chunk}
+                "chunk": cfg.chunk, "graph": cfg.graph}
 
-    def set_launch_config(self, pitch_streams=None, refine_streams=None, chunk=None) -> dict:
+    def set_launch_config(self, pitch_streams=None, refine_streams=None, chunk=None, graph=None) -> dict:
         """Changes the given fields of the launch shape (results do not
         depend on it); returns the new shape."""
         cur = self.launch_config()
@@ -331,6 +332,7 @@ class Placer:
             v = cur[k] if v is None else list(v)
             getattr(cfg, k)[0], getattr(cfg, k)[1] = int(v[0]), int(v[1])
         cfg.chunk = int(cur["chunk"] if chunk is None else chunk)
+        cfg.graph = int(cur["graph"] if graph is None else graph)
         check(lib().pp_launch_config_set(self._ctx, C.byref(cfg)), "launch_config_set")
         return self.launch_config()
 

@@ -9,13 +9,18 @@ pp_launch_config_set) and of the autotuner that sets it.
   and bit for bit, in both forms.
 * The tuner leaves a valid shape set, and placements under it equal the
   default shape's.
+* Graph mode (the pass's launches captured into a HIP graph, replayed by the
+  next passes with the same plan): every replay equals the eager results bit
+  for bit, with eager passes, other forms and other chunkings in between, so
+  a captured graph that went stale or a replay that read stale counters
+  would show.
 """
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 NQ = 3000
-DEFAULT = {"pitch_streams": [0, 8], "refine_streams": [4, 8], "chunk": 32768}
+DEFAULT = {"pitch_streams": [0, 8], "refine_streams": [4, 8], "chunk": 32768, "graph": 0}
 
 
 def _print_case(name):
@@ -48,7 +53,7 @@ def test_launch_config_validation(gtr50_3k):
     pl = gtr50_3k["pl"]
     _print_case("launch config validation")
     for bad in ({"pitch_streams": [9, 0]}, {"refine_streams": [0, -1]}, {"chunk": 0},
-                {"chunk": 262145}):
+                {"chunk": 262145}, {"graph": 2}):
         with pytest.raises(PPError):
             pl.set_launch_config(**bad)
         assert pl.launch_config() == DEFAULT
@@ -57,9 +62,10 @@ def test_launch_config_validation(gtr50_3k):
 
 
 @pytest.mark.parametrize("shape", [
-    {"pitch_streams": [0, 0], "refine_streams": [0, 0], "chunk": 1000},
-    {"pitch_streams": [2, 2], "refine_streams": [8, 2], "chunk": 1200},
-    {"pitch_streams": [8, 4], "refine_streams": [2, 0], "chunk": 65536},
+    {"pitch_streams": [0, 0], "refine_streams": [0, 0], "chunk": 1000, "graph": 0},
+    {"pitch_streams": [2, 2], "refine_streams": [8, 2], "chunk": 1200, "graph": 0},
+    {"pitch_streams": [8, 4], "refine_streams": [2, 0], "chunk": 65536, "graph": 0},
+    {"pitch_streams": [4, 8], "refine_streams": [4, 8], "chunk": 1000, "graph": 1},
 ])
 def test_results_do_not_depend_on_launch_shape(gtr50_3k, shape):
     from pplacer_amd.placer import PP_FP32, PP_FP64
@@ -87,5 +93,26 @@ def test_tuner_sets_a_valid_shape(gtr50_3k):
             assert len(trials) >= 2 and all(ms > 0 for _, ms in trials)
             got, status, _ = pl.place(queries, precision=prec)
             assert (_fields(got), list(status[:NQ])) == gtr50_3k["want"][form]
+    finally:
+        pl.set_launch_config(**DEFAULT)
+
+
+def test_graph_replays_equal_eager_passes(gtr50_3k):
+    from pplacer_amd.placer import PP_FP32, PP_FP64
+    pl, queries, want = gtr50_3k["pl"], gtr50_3k["queries"], gtr50_3k["want"]
+    try:
+        # FP64 form: capture, then replays with an eager pass, the other form
+        # and another chunking (another plan, captured beside) in between
+        for step, (form, shape) in enumerate([
+                ("fp64", dict(DEFAULT, graph=1)), ("fp64", dict(DEFAULT, graph=1)),
+                ("fp64", dict(DEFAULT, graph=0)), ("fp32", dict(DEFAULT, graph=1)),
+                ("fp64", dict(DEFAULT, graph=1)), ("fp64", dict(DEFAULT, graph=1, chunk=1000)),
+                ("fp32", dict(DEFAULT, graph=1)), ("fp64", dict(DEFAULT, graph=1))]):
+            _print_case("graph replay step %d: %s, %s" % (step, form, shape))
+            pl.set_launch_config(**shape)
+            got, status, stats = pl.place(queries, precision=PP_FP64 if form == "fp64" else PP_FP32)
+            assert (_fields(got), list(status[:NQ])) == want[form], step
+            if shape["graph"]:
+                assert stats.t_ml_ms > 0 and stats.t_pitch_ms == 0
     finally:
         pl.set_launch_config(**DEFAULT)
