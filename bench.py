@@ -25,7 +25,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
-FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (packed) and FP32 matrix (dense MFMA) peak
 
 
 WORKLOAD_TEXT = {
@@ -52,6 +52,13 @@ def pitch_flops(stats, R=4, S=4):
             + (4 * R * S * S + 4 * R * S) * stats.n_site_probes_dist_pitch)
 
 
+def mfma_flops(stats, R=4, S=20):
+    """Flops the FP32 20-state engine issues on the matrix cores: per masked
+    site of a distal probe, the proximal and distal evolves of every rate
+    (2 * R*S*S multiply-adds).  Pendant probes use no MFMA."""
+    return 4 * R * S * S * stats.n_site_probes_dist_pitch
+
+
 def _per(x, ms):
     """x per second over ms milliseconds (None without a time)."""
     return x / (ms * 1e-3) if ms > 0 else None
@@ -60,13 +67,14 @@ def _per(x, ms):
 def read_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size)."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        f = line.split()
-        if len(f) >= 2 and not line.startswith("#") and f[0] == kernel:
-            return float(f[1])
+    for fn in ("pmc_traffic.txt", "pmc_mfma_prot500_20k.txt"):
+        path = os.path.join(ROOT, "profiles", fn)
+        if not os.path.exists(path):
+            continue
+        for line in open(path):
+            f = line.split()
+            if len(f) >= 2 and not line.startswith("#") and f[0] == kernel:
+                return float(f[1])
     return None
 
 
@@ -243,12 +251,16 @@ def main():
             "stage2_tflops": _per(s2_flops / 1e12, st.t_pitch_ms + st.t_refine_ms),
             "pitch_u2_tflops": _per(pitch_flops(st, S=S) / 1e12, st.t_pitch_ms),
             "pitch_u2_tflops_per_launch": _per(pitch_flops(st, S=S) / 1e12, st.t_pitch_kernel_ms),
+            "pitch_mfma_tflops_per_launch": (_per(mfma_flops(st) / 1e12, st.t_pitch_kernel_ms)
+                                             if S == 20 and f == "fp32" else None),
             "t_graph_ms": st.t_ml_ms if args.graph else None,
             "pitch_launches": st.n_pitch_launches,
             "pitch_avg_launch_ms": st.t_pitch_kernel_ms / max(1, st.n_pitch_launches),
             "pitch_phase_ms_per_launch": st.t_pitch_ms / max(1, st.n_pitch_launches),
             "n_logdots": st.n_logdots, "n_like_calls": st.n_like_calls,
             "n_pitched": st.n_pitched, "n_launches": st.n_launches,
+            # FP32 form: queries placed again in FP64 after the prox_bl assertion
+            "n_fp64_replaced": st.n_fp64_replaced, "t_fp64_replace_ms": st.t_fp64_replace_ms,
             "launch": pl.launch_config(), "launch_tuned": bool(args.tune),
         }
     main_f = forms[args.precision]
@@ -258,7 +270,7 @@ def main():
                 ("refine_kernels", st_main["t_refine_ms"])], key=lambda kv: kv[1])[0]
     if args.graph:  # no phase times in graph mode
         kern = "pitch_kernel"
-    peak = FP64_PEAK_TFLOPS if (args.precision == "fp64" or S == 20) else FP32_PEAK_TFLOPS
+    peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
     real = "double" if args.precision == "fp64" else "float"
     if kern == "stage1_kernel":
         name = "stage1_kernel<%s,16>" % real
@@ -267,13 +279,25 @@ def main():
                     "frac": (st_main["stage1_gather_GBps"] or 0) / HBM_PEAK_GBS,
                     "traffic": read_traffic(name),
                     "note": "gathered table bytes; most are served by L1/L2"}
+    elif S == 20 and real == "float":
+        # the FP32 20-state baseball kernel evolves on the matrix cores
+        # (v_mfma_f32_4x4x1f32); achieved = its MFMA flops per launch
+        name = "pitch_prot_kernel<float,4>"
+        achieved = st_main["pitch_mfma_tflops_per_launch"]
+        roofline = {"bound": "mfma", "pipe": "MFMA fp32 (state-basis evolves, 4x4x1 16-block tiles)",
+                    "kernel": name, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                    "frac": achieved / peak if achieved else None, "traffic": read_traffic(name),
+                    "avg_launch_ms": st_main["pitch_avg_launch_ms"],
+                    "launches": st_main["pitch_launches"],
+                    "u2_achieved": st_main["pitch_u2_tflops_per_launch"],
+                    "work": "MFMA flops: 4*R*S^2 per masked site of a distal probe (both evolves, every rate)"}
     else:
-        # the 20-state form runs stage 2 in FP64 in both forms
         # 4 states: one FP64 site or two FP32 sites per worker lane; the FP64
         # kernel runs in the 512- and 768-lane launch classes (masked lengths
         # up to 448 and beyond), named together as MAXBD = *
         # (template arguments: real, rates, launch class, sites per lane,
-        # Price-CAT, FP64 site-order log sums)
+        # Price-CAT, FP64 site-order log sums); 20 states in FP64: the
+        # eigenbasis engine
         name = ("pitch_mw_kernel<double,4,*,1,false,true>" if S == 4 and real == "double"
                 else "pitch_mw_kernel<float,4,512,2,false,false>" if S == 4
                 else "pitch_prot_kernel<double,4>")

@@ -251,6 +251,12 @@ typedef struct {
   double t_pitch_kernel_ms;         /* sum over baseball launches of each launch's own
                                        duration (launches on concurrent streams overlap,
                                        so this can exceed t_pitch_ms) */
+  /* FP32 form: queries that failed the prox_bl >= 0 assertion (three_tax.ml:41-45)
+   * and were placed again in FP64, and the device time of that re-placement
+   * (its ranking and ML time, also counted in t_ml_ms; the per-kernel times
+   * and work counters above describe the FP32 pass alone) */
+  int64_t n_fp64_replaced;
+  double t_fp64_replace_ms;
 } pp_stats;
 
 /* Stage 1 for a batch: scores[q][e] = Glv.masked_logdot of the query (full

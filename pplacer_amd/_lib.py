@@ -49,7 +49,8 @@ class Stats(C.Structure):
                 ("n_launches", C.c_int64), ("t_table_ms", C.c_double),
                 ("n_site_probes_pend_pitch", C.c_int64), ("n_site_probes_dist_pitch", C.c_int64),
                 ("n_pitch_launches", C.c_int64), ("t_pp_ms", C.c_double),
-                ("t_pitch_kernel_ms", C.c_double)]
+                ("t_pitch_kernel_ms", C.c_double),
+                ("n_fp64_replaced", C.c_int64), ("t_fp64_replace_ms", C.c_double)]
 
 
 class LaunchConfig(C.Structure):

@@ -577,7 +577,11 @@ size_t stage2_lds_fp64(const pp_ctx *c, int bd, size_t taken, int spl, int strea
 // eigen coordinates of larger magnitude; in FP32 that cancellation loses
 // them, so 20-state stage 2 always runs in FP64 (the FP32 form keeps its
 // FP32 stage-1 table).
-bool stage2_fp32_ok(const pp_ctx *c) { return c->model.S == 4; }
+// The FP32 stage-2 engines: 4 states (all models), 20 states with gamma rates
+// (the state-basis MFMA form needs one rate per block of four pairs).
+bool stage2_fp32_ok(const pp_ctx *c) {
+  return c->model.S == 4 || (c->model.S == 20 && c->model.n_cats == 0 && (c->model.R == 1 || c->model.R == 4));
+}
 
 // Longest masked length of the register-resident 4-state classes whose
 // workgroup fits in LDS (FP64, the larger form, with the pitch kernel's flags).
@@ -615,10 +619,14 @@ int stage2_bucket(const pp_ctx *c, int M) {
   return M <= c->reg_limit ? bucket_of(M) : kLongBucket;
 }
 
+int ensure_prot_scratch(pp_ctx *c);
+
 // Product scratch of the streaming form: one slice of long_cap sites per
 // workgroup the device can hold at once (1024-lane blocks: at most two per
 // CU); sized once before a pass, so that no launch finds it reallocated.
+// The 20-state kernels' scratch is reserved here too.
 int ensure_stream_scratch(pp_ctx *c) {
+  if (c->model.S == 20) return ensure_prot_scratch(c);
   if (c->model.S != 4 || c->long_cap == 0) return PP_OK;
   const int blocks = 2 * c->n_cu;
   double *p;
@@ -728,35 +736,86 @@ int launch_marg(pp_ctx *c, OptArgs a, int b) {
   return launch_marg_b<R, 1024, 2, CAT, STRICT>(c, a, bd);
 }
 
-// 20-state stage 2: one master + 7 worker waves per query, pairs per lane as
-// the longest masked length needs, per-workgroup pair scratch.
-// Sized for the kernel's full-occupancy grid, so that launches with fewer
-// workgroups never reallocate it while earlier launches may still run.
-template <typename Real>
-int prot_scratch(pp_ctx *c, const void *kern, size_t lds, int cap_pairs, void **out) {
-  const int grid_full = occupancy_grid<Real, 4>(kern, kProtBlock, lds, c->n_cu, INT_MAX);
-  Real *p;
-  ENSURE(c->b_scratch, ProtLayout::scratch_reals(cap_pairs) * (size_t)grid_full, &p);
-  *out = p;
-  return PP_OK;
+// 20-state stage 2: one master + 7 worker waves per query.  FP64 form: pairs
+// per lane as the longest masked length needs, per-workgroup pair state (a, a,
+// G); FP32 form: 64-pair tiles, per-workgroup w of every pair.
+template <typename Real, int R>
+size_t prot_lds(const pp_ctx *c) {
+  if constexpr (std::is_same<Real, float>::value) return ProtLayout::align(Prot32Layout<R>::bytes(c->N));
+  return ProtLayout::align(ProtLayout::bytes(c->N));
 }
 
+template <typename Real, int R>
 int prot_cap_pairs(const pp_ctx *c) {
-  constexpr int worker_lanes = kProtBlock - 64;
   int max_m = 1;
   for (int q = 0; q < c->Q; q++)
     if (c->qstat[q] == PP_Q_OK) max_m = std::max(max_m, c->qM[q]);
-  const int pairs = max_m * c->model.R;
+  if constexpr (std::is_same<Real, float>::value) {
+    constexpr int st = Prot32Layout<R>::kTileSites;
+    return (max_m + st - 1) / st * 64;
+  }
+  constexpr int worker_lanes = kProtBlock - 64;
+  const int pairs = max_m * R;
   return (pairs + worker_lanes - 1) / worker_lanes * worker_lanes;
 }
 
 template <typename Real, int R>
+size_t prot_scratch_bytes(int cap_pairs) {
+  if constexpr (std::is_same<Real, float>::value)
+    return sizeof(float) * Prot32Layout<R>::scratch_floats(cap_pairs / 64);
+  return sizeof(double) * ProtLayout::scratch_reals(cap_pairs);
+}
+
+// Scratch of a 20-state kernel's full-occupancy grid.
+template <typename Real, int R>
+size_t prot_scratch_need(const pp_ctx *c, const void *kern, size_t lds) {
+  const int grid_full = occupancy_grid<Real, R>(kern, kProtBlock, lds, c->n_cu, INT_MAX);
+  return prot_scratch_bytes<Real, R>(prot_cap_pairs<Real, R>(c)) * (size_t)grid_full;
+}
+
+template <int R>
+size_t prot_scratch_max(const pp_ctx *c) {
+  const size_t lp32 = prot_lds<float, R>(c) + taken_bytes(c->E), lp64 = prot_lds<double, R>(c) + taken_bytes(c->E);
+  size_t n = 0;
+  n = std::max(n, prot_scratch_need<float, R>(c, reinterpret_cast<const void *>(&pitch_prot_kernel<float, R>), lp32));
+  n = std::max(n, prot_scratch_need<double, R>(c, reinterpret_cast<const void *>(&pitch_prot_kernel<double, R>), lp64));
+  n = std::max(n, prot_scratch_need<float, R>(c, reinterpret_cast<const void *>(&optimize_prot_kernel<float, R>),
+                                              prot_lds<float, R>(c)));
+  n = std::max(n, prot_scratch_need<double, R>(c, reinterpret_cast<const void *>(&optimize_prot_kernel<double, R>),
+                                               prot_lds<double, R>(c)));
+  n = std::max(n, prot_scratch_need<double, R>(c, reinterpret_cast<const void *>(&marginal_prot_kernel<R>),
+                                               prot_lds<double, R>(c)));
+  return n;
+}
+
+// The scratch every 20-state kernel of the current queries may use, allocated
+// before a pass, so that no launch (nor a captured graph) sees it move: the
+// kernels of one pass share the buffer and run on one stream.
+int ensure_prot_scratch(pp_ctx *c) {
+  if (c->model.S != 20) return PP_OK;
+  const size_t n = c->model.R == 4 ? prot_scratch_max<4>(c) : prot_scratch_max<1>(c);
+  char *p;
+  ENSURE(c->b_scratch, n, &p);
+  return PP_OK;
+}
+
+template <typename Real, int R>
+int prot_scratch(pp_ctx *c, const void *kern, size_t lds, int cap_pairs, void **out) {
+  const int grid_full = occupancy_grid<Real, R>(kern, kProtBlock, lds, c->n_cu, INT_MAX);
+  char *p;
+  const size_t bytes = prot_scratch_bytes<Real, R>(cap_pairs) * (size_t)grid_full;
+  ENSURE(c->b_scratch, bytes, &p);
+  *out = p;
+  return PP_OK;
+}
+
+template <typename Real, int R>
 int launch_pitch_prot(pp_ctx *c, PitchArgs a) {
-  const size_t lds = ProtLayout::align(ProtLayout::bytes(c->N)) + taken_bytes(c->E);
+  const size_t lds = prot_lds<Real, R>(c) + taken_bytes(c->E);
   const void *k = reinterpret_cast<const void *>(&pitch_prot_kernel<Real, R>);
   const int grid = occupancy_grid<Real, R>(k, kProtBlock, lds, c->n_cu, a.nq);
-  a.cap_pairs = prot_cap_pairs(c);
-  int st = prot_scratch<Real>(c, k, lds, a.cap_pairs, &a.scratch);
+  a.cap_pairs = prot_cap_pairs<Real, R>(c);
+  int st = prot_scratch<Real, R>(c, k, lds, a.cap_pairs, &a.scratch);
   if (st) return st;
   hipLaunchKernelGGL((pitch_prot_kernel<Real, R>), dim3(grid), dim3(kProtBlock), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
@@ -765,11 +824,11 @@ int launch_pitch_prot(pp_ctx *c, PitchArgs a) {
 
 template <typename Real, int R>
 int launch_opt_prot(pp_ctx *c, OptArgs a) {
-  const size_t lds = ProtLayout::bytes(c->N);
+  const size_t lds = prot_lds<Real, R>(c);
   const void *k = reinterpret_cast<const void *>(&optimize_prot_kernel<Real, R>);
   const int grid = occupancy_grid<Real, R>(k, kProtBlock, lds, c->n_cu, a.n_jobs);
-  a.cap_pairs = prot_cap_pairs(c);
-  int st = prot_scratch<Real>(c, k, lds, a.cap_pairs, &a.scratch);
+  a.cap_pairs = prot_cap_pairs<Real, R>(c);
+  int st = prot_scratch<Real, R>(c, k, lds, a.cap_pairs, &a.scratch);
   if (st) return st;
   hipLaunchKernelGGL((optimize_prot_kernel<Real, R>), dim3(grid), dim3(kProtBlock), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
@@ -778,11 +837,11 @@ int launch_opt_prot(pp_ctx *c, OptArgs a) {
 
 template <int R>
 int launch_marg_prot(pp_ctx *c, OptArgs a) {
-  const size_t lds = ProtLayout::bytes(c->N);
+  const size_t lds = prot_lds<double, R>(c);
   const void *k = reinterpret_cast<const void *>(&marginal_prot_kernel<R>);
   const int grid = occupancy_grid<double, R>(k, kProtBlock, lds, c->n_cu, a.n_jobs);
-  a.cap_pairs = prot_cap_pairs(c);
-  int st = prot_scratch<double>(c, k, lds, a.cap_pairs, &a.scratch);
+  a.cap_pairs = prot_cap_pairs<double, R>(c);
+  int st = prot_scratch<double, R>(c, k, lds, a.cap_pairs, &a.scratch);
   if (st) return st;
   hipLaunchKernelGGL((marginal_prot_kernel<R>), dim3(grid), dim3(kProtBlock), lds, c->cur, a);
   HIPCHECK(hipGetLastError());
@@ -1130,7 +1189,7 @@ int place_pass(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats)
                           d_counter, d_stats, d_tlo, d_thi, d_tq, c->b_tcodes.p, c->b_codes.p, c->b_qM.p,
                           c->b_lt.p, c->b_se.p, c->b_d64.p, c->b_p64.p, c->b_d32.p, c->b_p32.p, c->b_de.p,
                           c->b_pe.p, c->b_de32.p, c->b_pe32.p, c->b_bl.p, c->b_sscr.p, c->d_model, c->b_cat.p,
-                          c->b_prior.p};
+                          c->b_prior.p, c->b_scratch.p};
     for (const void *ptr : ptrs) h.add(ptr);
     h.add(c->reg_limit); h.add(c->long_cap); h.add(c->sscr_blocks);
     key = h.v;
@@ -1401,30 +1460,10 @@ int place_pass(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats)
     stats->t_pp_ms = t_pp;
     stats->t_pitch_kernel_ms = t_pitch_k;
     stats->n_launches = launches;
+    stats->n_fp64_replaced = 0;
+    stats->t_fp64_replace_ms = 0;
   }
   return PP_OK;
-}
-
-// Adds the counters and phase times of a second pass to a pass's stats.
-void add_stats(pp_stats *a, const pp_stats &b) {
-  a->t_ranking_ms += b.t_ranking_ms;
-  a->t_ml_ms += b.t_ml_ms;
-  a->n_logdots += b.n_logdots;
-  a->n_like_calls += b.n_like_calls;
-  a->n_pitched += b.n_pitched;
-  a->t_stage1_ms += b.t_stage1_ms;
-  a->t_pitch_ms += b.t_pitch_ms;
-  a->t_refine_ms += b.t_refine_ms;
-  a->n_site_pairs += b.n_site_pairs;
-  a->n_site_probes_pend += b.n_site_probes_pend;
-  a->n_site_probes_dist += b.n_site_probes_dist;
-  a->n_launches += b.n_launches;
-  a->t_table_ms += b.t_table_ms;
-  a->n_site_probes_pend_pitch += b.n_site_probes_pend_pitch;
-  a->n_site_probes_dist_pitch += b.n_site_probes_dist_pitch;
-  a->n_pitch_launches += b.n_pitch_launches;
-  a->t_pp_ms += b.t_pp_ms;
-  a->t_pitch_kernel_ms += b.t_pitch_kernel_ms;
 }
 
 // Queries of an FP32 pass that failed on the prox_bl >= 0 assertion
@@ -1491,7 +1530,11 @@ int replace_in_fp64(pp_ctx *c, const pp_prefs *prefs, const std::vector<int> &id
                      reinterpret_cast<int *>(c->b_status.p));
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(c->stream));
-  if (stats) add_stats(stats, sub);
+  if (stats) {
+    stats->n_fp64_replaced += n;
+    stats->t_fp64_replace_ms += sub.t_ranking_ms + sub.t_ml_ms;
+    stats->t_ml_ms += sub.t_ranking_ms + sub.t_ml_ms;
+  }
   return PP_OK;
 }
 

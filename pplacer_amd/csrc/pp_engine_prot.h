@@ -25,6 +25,7 @@
 #pragma once
 
 #include <climits>
+#include <type_traits>
 
 #include "pp_engine.h"
 #include "pp_engine_ctl.h"
@@ -41,6 +42,10 @@ struct ProtShared {
   // exp(rate_r * bl * l_k) at [r][k] for the proximal, distal, pendant lengths
   double ep[kMaxRates * kProtS], ed[kMaxRates * kProtS], en[kMaxRates * kProtS];
   int op, M, q, loc;
+  // FP32 form: which transition tables a probe rebuilds (bit 0: proximal and
+  // distal, bit 1: pendant), and the number of character codes
+  int tflags, n_codes;
+  int tile_next;  // FP32 form: the next 64-pair tile a worker wave takes
   double part[16];      // per-worker-wave sums of site logs
   long long ipart[16];  // per-worker-wave sums of site exponents
 };
@@ -218,6 +223,8 @@ struct ProtMaster : ThreeTaxControl<ProtMaster<Real, R>> {
   using Ctl::n_probes;
   using Ctl::n_dist_probes;
   static constexpr int S = kProtS;
+  // FP32 form: state-basis MFMA workers (pp_engine_prot32.h)
+  static constexpr bool kF32 = std::is_same<Real, float>::value;
   ProtShared *sh;
   int *sidx;
   int lane, n_workers, cap_sites, n_inf, n_sl;
@@ -239,6 +246,9 @@ struct ProtMaster : ThreeTaxControl<ProtMaster<Real, R>> {
     if (lane == 0) sh->op = op;
     __syncthreads();  // A
     if (op == CMD_EXIT) return;
+    if constexpr (kF32) {
+      if (op == CMD_PROBE || op == CMD_PROBE_DIST) __syncthreads();  // C: the workers built the FP32 tables
+    }
     __syncthreads();  // B
   }
 
@@ -306,6 +316,10 @@ struct ProtMaster : ThreeTaxControl<ProtMaster<Real, R>> {
     const bool new_pend = !(pend_bl == c_pend);
     const bool new_dist = !(dist_bl == c_dist && prox_bl == c_prox);
     tables(new_pend, new_dist);
+    if (lane == 0) {
+      sh->tflags = (new_dist ? 1 : 0) | (new_pend ? 2 : 0);
+      sh->tile_next = 0;
+    }
     command(new_dist ? CMD_PROBE_DIST : CMD_PROBE);
     double pv[16];
 #pragma unroll

@@ -4,6 +4,7 @@
 #include "pp_engine.h"
 #include "pp_engine_mw.h"
 #include "pp_engine_prot.h"
+#include "pp_engine_prot32.h"
 #include "pp_kernels.h"
 #include "pp_twin_kernels.h"
 
@@ -308,16 +309,53 @@ void pitch_mw_kernel(PitchArgs a) {
   pitch_loop(a, m, taken);
 }
 
+// The 20-state workers of either form: FP64 in the eigenbasis
+// (pp_engine_prot.h), FP32 in the state basis on the matrix cores
+// (pp_engine_prot32.h).  Returns the LDS byte past the engine's layout.
 template <typename Real, int R>
-__global__ __launch_bounds__(512) void pitch_prot_kernel(PitchArgs a) {
+__device__ __forceinline__ size_t prot_setup(char *smem, const DevModel *dm, int N) {
+  if constexpr (std::is_same<Real, float>::value) {
+    prot32_copy_model<R>(smem, dm, N);
+    __syncthreads();
+    return ProtLayout::align(Prot32Layout<R>::bytes(N));
+  } else {
+    prot_copy_model(reinterpret_cast<ProtShared *>(smem), dm);
+    __syncthreads();
+    return ProtLayout::align(ProtLayout::bytes(N));
+  }
+}
+
+template <typename Real, int R, class Args>
+__device__ __forceinline__ void prot_workers(char *smem, const Args &a) {
+  if constexpr (std::is_same<Real, float>::value) {
+    float *scr = reinterpret_cast<float *>(a.scratch) +
+                 Prot32Layout<R>::scratch_floats(a.cap_pairs / 64) * blockIdx.x;
+    prot32_worker<R>(smem, scr, a.N, a.codes, reinterpret_cast<const float *>(a.darr),
+                     reinterpret_cast<const float *>(a.parr), a.darr_e, a.parr_e);
+  } else {
+    double *scr = reinterpret_cast<double *>(a.scratch) + ProtLayout::scratch_reals(a.cap_pairs) * blockIdx.x;
+    prot_worker<double, R>(smem, scr, a.cap_pairs, a.N, a.codes, reinterpret_cast<const double *>(a.darr),
+                           reinterpret_cast<const double *>(a.parr), a.darr_e, a.parr_e, a.site_cat);
+  }
+}
+
+// Waves per SIMD the 20-state stage-2 kernels are built for: the FP64
+// engine's pair temporaries take the launch bound's budget (one workgroup per
+// CU); the FP32 form fits 128 VGPRs, so two workgroups share a CU and one's
+// table build and control overlap the other's matrix-core work.
+template <typename Real>
+struct ProtOcc {
+  static constexpr int waves = std::is_same<Real, float>::value ? 4 : 2;
+};
+
+template <typename Real, int R>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(ProtOcc<Real>::waves)))
+void pitch_prot_kernel(PitchArgs a) {
   extern __shared__ __align__(16) char smem[];
-  prot_copy_model(reinterpret_cast<ProtShared *>(smem), a.model);
-  __syncthreads();
-  unsigned *taken = reinterpret_cast<unsigned *>(smem + ProtLayout::align(ProtLayout::bytes(a.N)));
+  const size_t end = prot_setup<Real, R>(smem, a.model, a.N);
+  unsigned *taken = reinterpret_cast<unsigned *>(smem + end);
   if (wave_id() != 0) {
-    Real *scr = reinterpret_cast<Real *>(a.scratch) + ProtLayout::scratch_reals(a.cap_pairs) * blockIdx.x;
-    prot_worker<Real, R>(smem, scr, a.cap_pairs, a.N, a.codes, reinterpret_cast<const Real *>(a.darr),
-                         reinterpret_cast<const Real *>(a.parr), a.darr_e, a.parr_e, a.site_cat);
+    prot_workers<Real, R>(smem, a);
     return;
   }
   ProtMaster<Real, R> m;
@@ -478,14 +516,12 @@ void optimize_mw_kernel(OptArgs a) {
 }
 
 template <typename Real, int R>
-__global__ __launch_bounds__(512) void optimize_prot_kernel(OptArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(ProtOcc<Real>::waves)))
+void optimize_prot_kernel(OptArgs a) {
   extern __shared__ __align__(16) char smem[];
-  prot_copy_model(reinterpret_cast<ProtShared *>(smem), a.model);
-  __syncthreads();
+  (void)prot_setup<Real, R>(smem, a.model, a.N);
   if (wave_id() != 0) {
-    Real *scr = reinterpret_cast<Real *>(a.scratch) + ProtLayout::scratch_reals(a.cap_pairs) * blockIdx.x;
-    prot_worker<Real, R>(smem, scr, a.cap_pairs, a.N, a.codes, reinterpret_cast<const Real *>(a.darr),
-                         reinterpret_cast<const Real *>(a.parr), a.darr_e, a.parr_e, a.site_cat);
+    prot_workers<Real, R>(smem, a);
     return;
   }
   ProtMaster<Real, R> m;

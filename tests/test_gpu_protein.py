@@ -2,12 +2,12 @@
 through the C ABI.
 
 The tree passes and stage 1 use the stubs' arithmetic (GLVs bit-equal,
-stage-1 scores within 1e-12 relative).  Stage 2 evaluates the same
+stage-1 scores within 1e-12 relative).  The FP64 stage 2 evaluates the same
 likelihood in the eigenbasis of Q instead of forming P(t), so probe
-likelihoods agree to rounding; placements are held to the nucleotide path's
-bar: the same best edge (or a near-tie), likelihood within 1e-9 relative in
-FP64 and 1e-4 absolute in FP32 (refined in FP64), branch lengths within
-2e-4.
+likelihoods agree to rounding; the FP32 stage 2 evolves with FP32 P(t) on
+the matrix cores.  Placements are held to the nucleotide path's bar: the
+same best edge (or a near-tie), likelihood within 1e-9 relative in FP64 and
+1e-4 absolute in FP32 (refined in FP64), branch lengths within 2e-4.
 """
 import numpy as np
 import pytest
@@ -78,6 +78,41 @@ def test_protein_three_tax_optimize(prot10):
         assert st[j] == ost == 0
         assert abs(ll[j] - oll) <= 1e-9 * abs(oll), (j, ll[j], oll)
         assert abs(pend[j] - opend) < 2e-2 and abs(dist[j] - odist) < 2e-2
+
+
+def test_protein_fp32_mfma_optimize(prot10):
+    """The FP32 form's 20-state engine (state-basis evolves on the matrix
+    cores, pp_engine_prot32.h) on optimize jobs at the final tolerance
+    (1e-5, core.ml:14).  Its error against FP64 at the same branch lengths
+    (the job's last probe, whose lengths it reports) is held to the north
+    star's FP32 bar, 1e-4 absolute.  Its Brent path differs from the FP64
+    one, so its optimum is held to the FP32 bar plus what the likelihood's
+    FP32 noise lets Brent settle on: 5e-4 of the oracle's optimum, lengths
+    within 1e-2."""
+    from pplacer_amd.placer import PP_FP32
+    _print_case("prot10 three_tax optimize fp32 (MFMA)")
+    pl, core, queries = prot10["pl"], prot10["core"], prot10["queries"]
+    E = prot10["wl"].tree.n_nodes - 1
+    jobs = [(q, loc) for q in range(len(queries)) for loc in (0, E // 3, E // 2, E - 1)]
+    qi = [j[0] for j in jobs]
+    locs = [j[1] for j in jobs]
+    pend0 = [0.1] * len(jobs)
+    dist0 = [float(prot10["wl"].tree.bl[loc]) / 2 for loc in locs]
+    ll, pend, dist, calls, st = pl.optimize(queries, qi, locs, pend0, dist0, 1e-5, 2.0, PP_FP32)
+    e_eval, e_opt, e_len = [], [], []
+    for j, (q, loc) in enumerate(jobs):
+        ost, oll, opend, odist, ocalls = core.optimize(queries[q], loc, pend0[j], dist0[j], 1e-5)
+        assert st[j] == ost == 0
+        at = core.log_like(queries[q], loc, pend[j], dist[j])
+        e_eval.append(abs(ll[j] - at))
+        e_opt.append(abs(ll[j] - oll))
+        e_len.append(max(abs(pend[j] - opend), abs(dist[j] - odist)))
+    print("fp32 MFMA optimize, %d jobs: max |ll - ll_fp64(same lengths)| = %.3g, "
+          "max |ll - oracle optimum| = %.3g, max length difference %.3g"
+          % (len(jobs), max(e_eval), max(e_opt), max(e_len)), flush=True)
+    assert max(e_eval) <= 1e-4
+    assert max(e_opt) <= 5e-4
+    assert max(e_len) <= 1e-2
 
 
 def test_protein_place_fp64(prot10):
