@@ -614,8 +614,10 @@ int max_masked_sites(const pp_ctx *c) {
 
 // Stage-2 class of a query of masked length M (4 states): its 64-site class
 // when the register-resident form holds it, else the streaming class.
+// 20 states: one class (the kernel takes any masked length up to the
+// launch's capacity), so a chunk's stage 2 is one launch per phase.
 int stage2_bucket(const pp_ctx *c, int M) {
-  if (c->model.S == 20) return bucket_of(std::min(M, 64 * kLongBucket));
+  if (c->model.S == 20) return 0;
   return M <= c->reg_limit ? bucket_of(M) : kLongBucket;
 }
 
@@ -1104,6 +1106,11 @@ int place_pass(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats)
     std::vector<std::vector<int>> buckets(kBuckets);
     for (int l = 0; l < nq; l++)
       if (c->qstat[q0 + l] == PP_Q_OK) buckets[stage2_bucket(c, c->qM[q0 + l])].push_back(l);
+    // the single 20-state class: longest queries first, so that the launch's
+    // tail holds the short ones
+    if (c->model.S == 20)
+      std::stable_sort(buckets[0].begin(), buckets[0].end(),
+                       [&](int x, int y) { return c->qM[q0 + x] > c->qM[q0 + y]; });
     std::vector<int> &boff = boffs[ci];
     int n = 0;
     for (int b = 0; b < kBuckets; b++) {
