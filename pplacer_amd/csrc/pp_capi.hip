@@ -1519,8 +1519,14 @@ int replace_in_fp64(pp_ctx *c, const pp_prefs *prefs, const std::vector<int> &id
   c->qlo = sub_lo;
   c->qhi = sub_hi;
   c->qstat.assign(n, PP_Q_OK);
+  // A handful of queries spread over the stage-2 classes: their launches run
+  // concurrently on the FP32 form's streams rather than one after another.
+  const pp_launch_config launch = c->launch;
+  c->launch.pitch_streams[0] = launch.pitch_streams[1];
+  c->launch.refine_streams[0] = launch.refine_streams[1];
   pp_stats sub{};
   int st = place_pass(c, prefs, PP_FP64, &sub);
+  c->launch = launch;
   swap_bufs();
   c->Q = Q;
   c->order = std::move(order_s);
