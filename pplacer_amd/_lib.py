@@ -81,6 +81,8 @@ SIGNATURES = {
                                           C.c_void_p]),
     # test hook of the FP64 engine's wave-parallel sequential sum (not in the header)
     "pp_test_sequential_sum": (_i, [_dp, _i, _dp]),
+    "pp_test_cr_math": (_i, [C.c_longlong, _dp, _dp, _dp]),
+    "pp_test_cr_math_host": (_i, [C.c_longlong, _dp, _dp, _dp]),
     "pp_mat_log_like3": (_i, [_dp, _dp, _dp, _dp, _i, _i, _dp]),
     "pp_mat_pairwise_prod": (_i, [_dp, _dp, _dp, _i, _i]),
     "pp_mat_statd_pairwise_prod": (_i, [_dp, _dp, _dp, _dp, _i, _i]),

@@ -2208,6 +2208,29 @@ int pp_test_sequential_sum(const double *x, int n, double *out) {
   return PP_OK;
 }
 
+// Test hook, not part of the drop-in ABI: the correctly rounded exp and log
+// (pp_crmath.h) on the device, out_exp[i] = exp(x[i]), out_log[i] = log(x[i]).
+static __global__ void cr_math_test_kernel(const double *x, long long n, double *oe, double *ol) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    oe[i] = cr::exp(x[i]);
+    ol[i] = cr::log(x[i]);
+  }
+}
+
+int pp_test_cr_math(long long n, const double *x, double *out_exp, double *out_log) {
+  CHECK(n >= 0 && (n == 0 || (x && out_exp && out_log)), PP_EINVAL, "bad arguments");
+  if (n == 0) return PP_OK;
+  Scratch s;
+  double *dx = s.up(x, (size_t)n), *de = s.up<double>(nullptr, (size_t)n), *dl = s.up<double>(nullptr, (size_t)n);
+  CHECK(dx && de && dl, PP_ENOMEM, "device allocation failed");
+  const int grid = (int)std::min<long long>(4096, (n + 255) / 256);
+  hipLaunchKernelGGL(cr_math_test_kernel, dim3(grid), dim3(256), 0, 0, dx, n, de, dl);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpy(out_exp, de, (size_t)n * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(out_log, dl, (size_t)n * 8, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
 // The mat_* stubs: the rate-free [site][state] GLVs of Gcat_model; the same
 // kernels with one rate and no n log R term.
 int pp_mat_log_like3(const double *statd, const double *x, const double *y, const double *z, int n_sites,

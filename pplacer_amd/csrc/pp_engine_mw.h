@@ -255,7 +255,7 @@ __device__ __forceinline__ void mw_worker(char *smem, const MWLayout<Real, R, ST
         if constexpr (strict) {
 #pragma unroll
           for (int k = 0; k < SPL; k++)
-            if (active[k]) slog[my + k * stride] = Ops::logd(util[k]);
+            if (active[k]) slog[my + k * stride] = Ops::logs(util[k]);
           __syncthreads();  // every site's log is in slog
           if (wv == 0) {
             const double t = wave_sequential_sum(slog, sh->M, lay.sacc(smem), lay.sev(smem), nullptr, kSeqSumWaveMin);
@@ -348,7 +348,7 @@ __device__ __forceinline__ void mw_worker_stream(char *smem, const MWLayout<Real
               util = t == 0 ? w * z[t] : Ops::madd(util, w, z[t]);
             }
           }
-          if constexpr (strict) slog[i] = Ops::logd(util);
+          if constexpr (strict) slog[i] = Ops::logs(util);
           else lsum += Ops::logd(util);
         }
         if constexpr (strict) {
@@ -507,7 +507,7 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT, STRICT>> {
       // exp(rate_r * bl * l_k) for prox (u < 4R) and dist (4R <= u < 8R)
       if (lane < 8 * R) {
         const int which = lane / (4 * R), r = (lane / 4) % R, k = lane % 4;
-        sE[lane] = exp((sh->rates[r] * (which ? dist_bl : prox_bl)) * sh->l[k]);
+        sE[lane] = texp((sh->rates[r] * (which ? dist_bl : prox_bl)) * sh->l[k]);
       }
       wave_lds_sync();
 #pragma unroll
@@ -526,7 +526,7 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT, STRICT>> {
       double *sEp = sE + 8 * R;
       if (lane < 4 * R) {
         const int r = lane / 4, k = lane % 4;
-        sEp[lane] = exp((sh->rates[r] * pend_bl) * sh->l[k]);
+        sEp[lane] = texp((sh->rates[r] * pend_bl) * sh->l[k]);
       }
       wave_lds_sync();
       // P_pend[r][i][j] at lane r*16 + i*4 + j (R*16 <= 64)
@@ -564,6 +564,14 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT, STRICT>> {
   // LDS (more entries than lanes): exponentials, then P(prox), P(dist) and
   // P(pend) entries; the one-hot pendant rows are P(pend) columns and the
   // ambiguity rows are formed from them.
+  // The exponentials of P(t): correctly rounded where the site logs are
+  // summed in site order (the FP64 form), as the stubs' libm returns them;
+  // the library's otherwise.
+  static __device__ __forceinline__ double texp(double v) {
+    if constexpr (STRICT) return cr::exp_call(v);
+    else return exp(v);
+  }
+
   __device__ __forceinline__ void cat_tables(bool new_pend, bool new_dist) {
     const int nd = new_dist ? 2 * n_sl * 4 : 0, np = new_pend ? n_sl * 4 : 0;
     // sE[0 .. 2 n_sl 4): prox then dist; sE[2 n_sl 4 ..): pend
@@ -572,7 +580,7 @@ struct MWMaster : ThreeTaxControl<MWMaster<Real, R, SPL, CAT, STRICT>> {
       const int u = pend ? t - nd : t;
       const int which = pend ? 2 : u / (n_sl * 4), rem = u % (n_sl * 4), s = rem / 4, k = rem % 4;
       const double bl = which == 0 ? prox_bl : which == 1 ? dist_bl : pend_bl;
-      sE[which * n_sl * 4 + rem] = exp((sh->rates[s] * bl) * sh->l[k]);
+      sE[which * n_sl * 4 + rem] = texp((sh->rates[s] * bl) * sh->l[k]);
     }
     wave_lds_sync();
     const int ndp = new_dist ? 2 * n_sl * 16 : 0, npp = new_pend ? n_sl * 16 : 0;

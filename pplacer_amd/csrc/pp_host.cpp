@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "pp_crmath.h"
 #include "pp_internal.h"
 #include "pp_tree.h"
 
@@ -421,6 +422,17 @@ int pp_prior_mean_map(int n_nodes, const int *child_start, const int *children,
   }
   pp::TreeDesc t{n_nodes, 0, child_start, children, bl};
   pp::prior_mean_map(t, lower, means);
+  return PP_OK;
+}
+
+// Test hook, not part of the drop-in ABI: the correctly rounded exp and log
+// of pp_crmath.h evaluated on the host (the same source the kernels use).
+int pp_test_cr_math_host(long long n, const double *x, double *out_exp, double *out_log) {
+  if (n < 0 || (n > 0 && (!x || !out_exp || !out_log))) return PP_EINVAL;
+  for (long long i = 0; i < n; i++) {
+    out_exp[i] = pp::cr::exp(x[i]);
+    out_log[i] = pp::cr::log(x[i]);
+  }
   return PP_OK;
 }
 

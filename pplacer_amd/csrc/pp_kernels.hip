@@ -42,7 +42,8 @@ __global__ __launch_bounds__(256) void code_log_kernel(CodeLogArgs a) {
     double util = 0.0;
 #pragma unroll
     for (int t = 0; t < RS; t++) util = util + z[t] * s[t];
-    const double l = log(util);
+    // the FP64 table's logs correctly rounded, as the stubs' libm's
+    const double l = a.lt32 ? log(util) : cr::log(util);
     if (a.lt32) a.lt32[base + (size_t)c * a.Epad] = (float)l;
     else a.lt64[base + (size_t)c * a.Epad] = l;
   }

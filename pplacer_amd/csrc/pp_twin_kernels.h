@@ -8,6 +8,8 @@
 
 #include <cstdint>
 
+#include "pp_crmath.h"
+
 namespace pp {
 
 // per-site util of ten_log_like3_c (statd != null) or ten_*_logdot_c;
@@ -31,7 +33,8 @@ static __global__ void util_kernel(double *util, const double *statd, const doub
   util[t] = u;
 }
 
-// sequential site-order log sum (linear_c.c:416-421 / 527-536); the mat_*
+// sequential site-order log sum (linear_c.c:416-421 / 527-536) of correctly
+// rounded logs (pp_crmath.h), as the stubs' libm returns them; the mat_*
 // stubs (linear_c.c:170-340) have no n log R term (rate_term = 0)
 static __global__ void logsum_kernel(const double *util, const uint16_t *mask, int first, int n_used,
                                      int R, int rate_term, double *out) {
@@ -39,8 +42,8 @@ static __global__ void logsum_kernel(const double *util, const uint16_t *mask, i
   double ll = 0;
   int used = 0;
   for (int t = 0; t < n_used; t++)
-    if (!mask || mask[first + t]) { ll += log(util[t]); used++; }
-  if (rate_term) ll -= ((double)(float)used) * log((double)(float)R);
+    if (!mask || mask[first + t]) { ll += cr::log(util[t]); used++; }
+  if (rate_term) ll -= ((double)(float)used) * cr::log((double)(float)R);
   *out = ll;
 }
 

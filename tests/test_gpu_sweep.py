@@ -5,15 +5,15 @@ every one compared with the oracle, in both forms.
   fails 5 queries on the prox_bl assertion, and so does the GPU, in both
   forms: the FP32 form places its own assertion failures again in FP64);
 * FP64: the kernels sum each probe's per-site logs in site order as the stub
-  does (linear_c.c:416-419), so a probe's value is the oracle's to the last bit
-  unless one of its exponentials or logarithms differs by an ulp from the host
-  libm's (measured on MI355X: 4.4% of exp and 0.34% of log results).  Measured:
-  1,730 of 1,995 placements at 50 taxa and 299 of 300 at 2,000 taxa bit-equal
-  to the oracle; where an ulp flips a Brent comparison at a flat optimum the
-  run ends elsewhere within the tolerance.  Bars: at least 99.9% within 1e-9
-  relative and all within 2e-9; at least 99.5% of the branch lengths within
-  twice final_tolerance (core.ml:14) and all within 1e-4 (a flat direction of
-  the likelihood lets two runs that part stop 5e-5 apart);
+  does (linear_c.c:416-419), with correctly rounded exp and log
+  (pplacer_amd/csrc/pp_crmath.h), so a probe's value is the oracle's to the
+  last bit unless the host libm misrounds one of its exponentials or
+  logarithms (glibc: well under 0.1% of calls).  Measured: 1,984 of 1,995
+  placements at 50 taxa and 300 of 300 at 2,000 taxa bit-equal to the oracle,
+  the rest within 1.7e-10 relative, where such an ulp flipped a Brent
+  comparison at a flat optimum.  Bars: every placement within 1e-9 relative,
+  every branch length within twice final_tolerance (core.ml:14), at least
+  99% (50 taxa) and 99.5% (2,000 taxa) bit-equal;
 * FP32: at least 99.9% within 1e-4 absolute and all within 2e-4 (measured
   maximum 1.02e-4: a boundary optimum, where the likelihood's slope turns the
   Brent tolerance into 1e-4), branch lengths within 2e-4.
@@ -68,8 +68,8 @@ def test_wide_sample_against_the_oracle(oracle, name, seed, nq):
     out = _sweep(oracle, name, seed, nq)
     rel, ab, bl = out["fp64"]
     n = len(rel)
-    assert sum(r <= 1e-9 for r in rel) >= 0.999 * n and max(rel) <= 2e-9
-    assert sum(b <= 2e-5 for b in bl) >= 0.995 * n and max(bl) <= 1e-4
-    assert sum(r == 0 for r in rel) >= (0.85 if name == "gtr50" else 0.99) * n
+    assert max(rel) <= 1e-9
+    assert max(bl) <= 2e-5
+    assert sum(r == 0 for r in rel) >= (0.99 if name == "gtr50" else 0.995) * n
     rel, ab, bl = out["fp32"]
     assert sum(a <= 1e-4 for a in ab) >= 0.999 * len(ab) and max(ab) <= 2e-4 and max(bl) <= 2e-4
