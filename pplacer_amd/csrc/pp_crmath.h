@@ -175,17 +175,23 @@ PP_CR_FN double log(double x) {
   // u^3/3
   const DD c3 = two_prod(u3.hi, pp_cr_log_pdd[4]);
   const double c3lo = c3.lo + (u3.hi * pp_cr_log_pdd[5] + u3.lo * pp_cr_log_pdd[4]);
-  double pp = pp_cr_log_p[15];
-  for (int k = 14; k >= 3; k--) pp = pp_cr_log_p[k] + u * pp;
-  const double r4 = (u2.hi * u2.hi) * pp;  // u^4 P(u)
-  // e ln2 + L + u - u^2/2 + u^3/3 + u^4 P(u)
-  DD s = two_sum(ef * pp_cr_ln2[0], l.hi);
+  // P(u) = sum_{k=3..15} c_k u^(k-3) by Estrin's scheme (four levels)
+  const double *c = pp_cr_log_p;
+  const double w2 = u2.hi, w4 = w2 * w2, w8 = w4 * w4;
+  const double q0 = (c[3] + u * c[4]) + w2 * (c[5] + u * c[6]);
+  const double q1 = (c[7] + u * c[8]) + w2 * (c[9] + u * c[10]);
+  const double q2 = (c[11] + u * c[12]) + w2 * (c[13] + u * c[14]);
+  const double pp = (q0 + w4 * q1) + w8 * (q2 + w4 * c[15]);
+  const double r4 = w4 * pp;  // u^4 P(u)
+  // e ln2 + L + u - u^2/2 + u^3/3 + u^4 P(u): |e ln2| >= ln2 > |L|, |L| >= 2^-8
+  // or L = 0 (and then |u| >= 2^-53 |u^2|): only u's term needs a two_sum
+  DD s = fast_two_sum(ef * pp_cr_ln2[0], l.hi);
   double lo = s.lo;
   s = two_sum(s.hi, u);
   lo += s.lo;
-  s = two_sum(s.hi, -0.5 * u2.hi);
+  s = fast_two_sum(s.hi, -0.5 * u2.hi);
   lo += s.lo;
-  s = two_sum(s.hi, c3.hi);
+  s = fast_two_sum(s.hi, c3.hi);
   lo += s.lo;
   // ul / (1 + u) for the part of u past one double
   lo += ((ef * pp_cr_ln2[1] + ef * pp_cr_ln2[2]) + l.lo) + ((-0.5 * u2.lo + c3lo) + r4) + (ul - ul * u);
