@@ -45,7 +45,7 @@ using namespace pp;
 namespace {
 
 constexpr int kTQ = 16;           // queries per stage-1 tile
-constexpr int kChunk = 32768;     // queries per device batch
+constexpr int kChunk = 65536;     // queries per device batch
 constexpr int kLongBucket = 30;   // stage-2 queries beyond the register-resident classes
 constexpr int kBuckets = 31;      // stage-2 masked-length classes of 64 sites, then kLongBucket
 constexpr int kBucketStreams = PP_MAX_BUCKET_STREAMS;  // streams for a stage-2 phase's buckets
@@ -472,21 +472,40 @@ int build_code_logs(pp_ctx *c, double start_pend) {
   return PP_OK;
 }
 
-// Stage-1 tiles of queries [q0, q0 + nq) (sorted order): kTQ queries per
-// tile (chunk-relative indices, -1 for empty or failed slots) and the span of
-// columns their informative characters cover, appended to lo/hi/q.
+// Stage-1 tiles of queries [q0, q0 + nq): kTQ queries per tile
+// (chunk-relative indices, -1 for empty slots and failed queries) and the
+// span of columns their informative characters cover, appended to lo/hi/q.
+// A tile's kernel walks every column of its span, so the tiles group queries
+// of similar spans: ordered by start column in windows of W columns (W sized
+// so that a window holds about 128 queries), then by end column.  A tile
+// then spans about its longest query plus W columns instead of most of the
+// alignment, and tiles next to one another read the same table columns.
+// Each (query, edge) sum is the same whichever tile holds the query.
 void stage1_tiles(const pp_ctx *c, int q0, int nq, std::vector<int> &lo_v, std::vector<int> &hi_v,
                   std::vector<int> &q_v) {
   const int N = c->N;
   const int n_tiles = (nq + kTQ - 1) / kTQ;
+  std::vector<int> ok;
+  ok.reserve(nq);
+  for (int l = 0; l < nq; l++)
+    if (c->qstat[q0 + l] == PP_Q_OK) ok.push_back(l);
+  const long long n_ok = std::max<long long>(1, (long long)ok.size());
+  const long long W = std::max(1LL, ((long long)N * 128 + n_ok - 1) / n_ok);
+  std::stable_sort(ok.begin(), ok.end(), [&](int x, int y) {
+    const long long wx = c->qlo[q0 + x] / W, wy = c->qlo[q0 + y] / W;
+    if (wx != wy) return wx < wy;
+    return c->qhi[q0 + x] < c->qhi[q0 + y];
+  });
   for (int t = 0; t < n_tiles; t++) {
     int lo = N, hi = 0;
     for (int j = 0; j < kTQ; j++) {
-      const int l = t * kTQ + j;
-      const int q = q0 + l;
-      const bool ok = l < nq && c->qstat[q] == PP_Q_OK;
-      q_v.push_back(ok ? l : -1);
-      if (!ok) continue;
+      const size_t i = (size_t)t * kTQ + j;
+      if (i >= ok.size()) {
+        q_v.push_back(-1);
+        continue;
+      }
+      const int l = ok[i], q = q0 + l;
+      q_v.push_back(l);
       lo = std::min(lo, c->qlo[q]);
       hi = std::max(hi, c->qhi[q]);
     }
@@ -1145,11 +1164,14 @@ int place_pass(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pitch_ev;
   // Runs launch(b) for every non-empty bucket, round-robin over the first
   // n_streams bucket streams (none: the main stream), and joins them back
-  // into the main stream.
+  // into the main stream.  Buckets go longest queries first: streams share
+  // the device's few hardware queues, so a launch made late may wait behind
+  // another on its queue, and the phase then ends on a short bucket's tail
+  // rather than a long bucket's whole run.
   auto phase = [&](const std::vector<int> &boff, int n_streams, auto &&launch) -> int {
     if (n_streams <= 0) {
       ls.set(c->stream);
-      for (int b = 0; b < kBuckets; b++)
+      for (int b = kBuckets - 1; b >= 0; b--)
         if (boff[b + 1] > boff[b]) {
           const int r = launch(b);
           if (r) return r;
@@ -1161,7 +1183,7 @@ int place_pass(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats)
     HIPCHECK(hipEventRecord(ef, c->stream));
     bool used[kBucketStreams] = {};
     int k = 0;
-    for (int b = 0; b < kBuckets; b++) {
+    for (int b = kBuckets - 1; b >= 0; b--) {
       if (boff[b + 1] == boff[b]) continue;
       const int si = k++ % std::min(n_streams, kBucketStreams);
       if (!used[si]) {

@@ -1,5 +1,5 @@
-"""GPU parity of a placement pass longer than one device chunk (32,768
-queries).  Such a pass runs as a pipeline: chunks of even size, stage 1 of the
+"""GPU parity of a placement pass longer than one device chunk (set to 32,768
+queries here).  Such a pass runs as a pipeline: chunks of even size, stage 1 of the
 next chunk on its own stream into the second score buffer while stage 2 of the
 current one runs, and (in the 4-state form) concurrent buckets.  Per-query
 results must not depend on how a pass is cut into chunks.
@@ -32,6 +32,7 @@ def long_pass():
     _print_case("workload gtr50 seed 5, %d queries" % NQ)
     wl = synth.make_workload("gtr50", seed=5, n_queries=NQ)
     pl, queries, statd, rates = placer.setup_from_workload(wl)
+    pl.set_launch_config(chunk=32768)
     yield dict(wl=wl, pl=pl, queries=queries)
     pl.close()
 
