@@ -54,7 +54,7 @@ static_assert(kChunk <= PP_MAX_CHUNK, "default chunk out of range");
 // main stream only), by form [FP64, FP32].  Measured at 100,000 queries,
 // 2,000 taxa.
 constexpr int kPitchStreams[2] = {0, 8};
-constexpr int kRefineStreams[2] = {4, 8};
+constexpr int kRefineStreams[2] = {4, 4};
 constexpr int kProtBlock = 512;   // lanes of a 20-state stage-2 workgroup (master + workers)
 
 struct DBuf {
@@ -1095,12 +1095,13 @@ int place_pass(pp_ctx *c, const pp_prefs *prefs, int precision, pp_stats *stats)
   const bool fp32 = precision == PP_FP32;
   const bool s2_fp32 = fp32 && stage2_fp32_ok(c);
   // Concurrent buckets, measured at 100,000 queries: the FP32 form 323 -> 274
-  // ms per pass with 4 streams, 258 -> 251 -> 245 ms with 8 for its baseball
-  // and then for its refinement; the FP64 refinement 364 -> 356 ms with 4 (8:
-  // no further gain).  The FP64 baseball kernels stay on the main stream by
-  // default, so that each launch's duration is the kernel's own; 8 streams
-  // measured 360 -> 346 ms on one host (2 or 4: no gain), and a tuner may set
-  // them.  The 20-state kernels share one scratch buffer and do not fork.
+  // ms per pass with 4 streams, 258 -> 251 ms with 8 for its baseball; the
+  // refinement with 4 streams (FP64 364 -> 356 ms; with 50,000-query chunks
+  // the FP32 form's 273-279 ms with 8 -> 267-270 ms with 4).  The FP64
+  // baseball kernels stay on the main stream by default, so that each
+  // launch's duration is the kernel's own (4 streams measured within 1% at
+  // 50,000-query chunks), and a tuner may set them.  The 20-state kernels
+  // share one scratch buffer and do not fork.
   const int pitch_streams = c->model.S == 4 ? c->launch.pitch_streams[s2_fp32 ? 1 : 0] : 0;
   const int refine_streams = c->model.S == 4 ? c->launch.refine_streams[s2_fp32 ? 1 : 0] : 0;
   hipEvent_t e_t0 = pool.get(), e_t1 = pool.get();

@@ -20,7 +20,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 NQ = 3000
-DEFAULT = {"pitch_streams": [0, 8], "refine_streams": [4, 8], "chunk": 65536, "graph": 0}
+DEFAULT = {"pitch_streams": [0, 8], "refine_streams": [4, 4], "chunk": 65536, "graph": 0}
 
 
 def _print_case(name):
