@@ -542,8 +542,12 @@ int launch_stage1_tiles(pp_ctx *c, int q0, int nq, double *scores, const int *d_
   a.qM = reinterpret_cast<const int *>(c->b_qM.p) + q0;
   a.log_R = std::log((double)(float)R);
   a.scores = scores;
-  dim3 grid(c->Epad / 256, n_tiles);
-  hipLaunchKernelGGL((stage1_kernel<Real, kTQ>), grid, dim3(256), 0, c->cur, a);
+  a.n_tiles = n_tiles;
+  const long long items = (long long)n_tiles * (c->Epad / 256);
+  CHECK(items < (1LL << 30), PP_EINVAL, "stage-1 grid too large");
+  a.per_xcd = (int)((items + 7) / 8);
+  if (items == 0) return PP_OK;
+  hipLaunchKernelGGL((stage1_kernel<Real, kTQ>), dim3(8 * a.per_xcd), dim3(256), 0, c->cur, a);
   HIPCHECK(hipGetLastError());
   return PP_OK;
 }

@@ -35,6 +35,8 @@ struct Stage1Args {
   const int *qM;          // masked length per (batch-local) query
   double log_R;
   double *scores;         // [Q][E]
+  int n_tiles;            // query tiles; the grid covers n_tiles x Epad/256 workgroups
+  int per_xcd;            // workgroups per XCD: ceil(n_tiles * Epad/256 / 8)
 };
 
 struct PitchArgs {

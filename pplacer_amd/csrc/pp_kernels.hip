@@ -66,11 +66,20 @@ template <> struct RowLoad<float> {
 };
 constexpr int kBufferDword3 = 0x00020000;  // gfx9 raw buffer resource word 3
 
+// The grid is one-dimensional, 8 * per_xcd workgroups.  The device deals
+// workgroups to its 8 XCDs round robin, so workgroup b runs on XCD b % 8;
+// XCD k takes the work items [k * per_xcd, (k + 1) * per_xcd) of the order
+// (edge tile, query tile), query tile fastest.  The workgroups resident on
+// one XCD at a time then share an edge tile and neighbouring query tiles,
+// whose spans overlap (the host plan groups queries by span), so they read
+// the same table rows through that XCD's L2.  Items past the end exit.
 template <typename Real, int TQ>
 __global__ __launch_bounds__(256) void stage1_kernel(Stage1Args a) {
   static_assert(TQ % 4 == 0, "codes are read four per dword");
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // < Epad by grid construction
-  const int tile = blockIdx.y;
+  const int item = (int)(blockIdx.x % 8u) * a.per_xcd + (int)(blockIdx.x / 8u);
+  if (item >= a.n_tiles * (a.Epad / 256)) return;
+  const int tile = item % a.n_tiles;
+  const int e = (item / a.n_tiles) * 256 + threadIdx.x;  // < Epad
   const int lo = a.tile_lo[tile], hi = a.tile_hi[tile];
   const unsigned gap = (unsigned)(a.C1 - 1);
   double ll[TQ];
