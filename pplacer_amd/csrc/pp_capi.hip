@@ -475,12 +475,13 @@ int build_code_logs(pp_ctx *c, double start_pend) {
 // Stage-1 tiles of queries [q0, q0 + nq): kTQ queries per tile
 // (chunk-relative indices, -1 for empty slots and failed queries) and the
 // span of columns their informative characters cover, appended to lo/hi/q.
-// A tile's kernel walks every column of its span, so the tiles group queries
-// of similar spans: ordered by start column in windows of W columns (W sized
-// so that a window holds about 128 queries), then by end column.  A tile
-// then spans about its longest query plus W columns instead of most of the
-// alignment, and tiles next to one another read the same table columns.
-// Each (query, edge) sum is the same whichever tile holds the query.
+// A tile's kernel walks every column of its span.  The resident queries are
+// stored by start column, so 16 consecutive ones start together but their
+// tile spans the longest of them; here they are ordered by start column in
+// windows of W columns (W sized so that a window holds about 128 queries),
+// then by end column, so that a tile's queries also end together and its
+// span is about their common length plus W.  Each (query, edge) sum is the
+// same whichever tile holds the query.
 void stage1_tiles(const pp_ctx *c, int q0, int nq, std::vector<int> &lo_v, std::vector<int> &hi_v,
                   std::vector<int> &q_v) {
   const int N = c->N;
@@ -490,12 +491,20 @@ void stage1_tiles(const pp_ctx *c, int q0, int nq, std::vector<int> &lo_v, std::
   for (int l = 0; l < nq; l++)
     if (c->qstat[q0 + l] == PP_Q_OK) ok.push_back(l);
   const long long n_ok = std::max<long long>(1, (long long)ok.size());
-  const long long W = std::max(1LL, ((long long)N * 128 + n_ok - 1) / n_ok);
-  std::stable_sort(ok.begin(), ok.end(), [&](int x, int y) {
-    const long long wx = c->qlo[q0 + x] / W, wy = c->qlo[q0 + y] / W;
-    if (wx != wy) return wx < wy;
-    return c->qhi[q0 + x] < c->qhi[q0 + y];
-  });
+  const int W = (int)std::max(1LL, ((long long)N * 128 + n_ok - 1) / n_ok);
+  {
+    // two stable counting sorts (end column, then window): the order of a
+    // stable sort by (window, end), in O(queries + N)
+    const int n_win = (N - 1) / W + 1;
+    std::vector<int> tmp(ok.size()), cnt((size_t)std::max(N + 1, n_win) + 1, 0);
+    for (int l : ok) cnt[c->qhi[q0 + l] + 1]++;
+    for (int i = 1; i <= N + 1; i++) cnt[i] += cnt[i - 1];
+    for (int l : ok) tmp[cnt[c->qhi[q0 + l]]++] = l;
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int l : tmp) cnt[c->qlo[q0 + l] / W + 1]++;
+    for (int i = 1; i <= n_win; i++) cnt[i] += cnt[i - 1];
+    for (int l : tmp) ok[cnt[c->qlo[q0 + l] / W]++] = l;
+  }
   for (int t = 0; t < n_tiles; t++) {
     int lo = N, hi = 0;
     for (int j = 0; j < kTQ; j++) {
