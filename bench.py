@@ -172,6 +172,11 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="capture each form's placement pass into a HIP graph and replay it "
                          "(no per-phase or per-launch times are recorded then)")
+    ap.add_argument("--fp32-graph", action=argparse.BooleanOptionalAction, default=True,
+                    help="when the FP32 form runs beside the FP64 one (4 states), run it as the "
+                         "batched form: its pass captured into a HIP graph in the warmup and "
+                         "replayed in the timed steps (the FP64 form stays eager, so that its "
+                         "per-launch times feed the roofline)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size "
                          "kernel for scaling hardware byte counters)")
@@ -208,8 +213,13 @@ def main():
         log("copy calibration: %d MiB read + %d MiB written" % (args.copy_calibration_mb,
                                                                args.copy_calibration_mb))
 
+    # the FP32 form as the other form of a 4-state run: replayed from a graph
+    fp32_graph = (args.fp32_graph and not args.graph and not args.tune and S == 4
+                  and args.precision == "fp64" and not args.calc_pp)
+
     def run_form(prec_name):
         prec = PP_FP64 if prec_name == "fp64" else PP_FP32
+        pl.set_launch_config(graph=1 if (args.graph or (fp32_graph and prec_name == "fp32")) else 0)
         if args.tune:
             from pplacer_amd import tune
             best, trials = tune.tune(pl, prefs, prec, log=log)
@@ -253,7 +263,7 @@ def main():
             "pitch_u2_tflops_per_launch": _per(pitch_flops(st, S=S) / 1e12, st.t_pitch_kernel_ms),
             "pitch_mfma_tflops_per_launch": (_per(mfma_flops(st) / 1e12, st.t_pitch_kernel_ms)
                                              if S == 20 and f == "fp32" else None),
-            "t_graph_ms": st.t_ml_ms if args.graph else None,
+            "t_graph_ms": st.t_ml_ms if pl.launch_config()["graph"] else None,
             "pitch_launches": st.n_pitch_launches,
             "pitch_avg_launch_ms": st.t_pitch_kernel_ms / max(1, st.n_pitch_launches),
             "pitch_phase_ms_per_launch": st.t_pitch_ms / max(1, st.n_pitch_launches),
